@@ -1,0 +1,224 @@
+/*
+ * tsx.h -- C ABI of the MI355X-native batched classical force-and-step path.
+ *
+ * One shared library (libtsx.so, gfx950) exports every entry point below.
+ * Conventions shared by all functions:
+ *   - All pointers are device pointers allocated by the caller (PyTorch);
+ *     the library never allocates or frees user memory.  Scratch space comes
+ *     in through explicit workspace arguments sized by the *_ws_bytes calls.
+ *   - ``stream`` is a hipStream_t passed as void*.  Work is enqueued
+ *     asynchronously on it; nothing synchronizes the device, nothing reads
+ *     back to the host, so every call is capturable in a hipGraph.
+ *   - Return value: 0 on success, <0 on a bad argument (TSX_ERR_ARG) or a HIP
+ *     error (TSX_ERR_HIP); tsx_last_error() returns a thread-local message.
+ *     Capacity problems discovered on the device (neighbor-list or bin
+ *     overflow) are reported through the int32 ``status`` word that the
+ *     caller reads when it synchronizes (TSX_STATUS_* bits).
+ *   - Cells are the column-vector SimState cells [S,3,3] (lattice vector k is
+ *     column k), float64.  Positions are float64 [N,3] (stride 3) or the
+ *     packed engine layout (stride 4).  Atoms of one system are contiguous;
+ *     ``sys_start`` [S+1] (int32) gives the atom range of each system and
+ *     ``atom_sys`` [N] (int32) the system of each atom.
+ *
+ * Reference interfaces replaced (TorchSim v0.6.1, paths relative to the
+ * reference repository):
+ *   tsx_nl_build / tsx_nl_count_pairs / tsx_nl_emit_pairs
+ *       torch_sim/neighbors/__init__.py:46-98 (torchsim_nl) and
+ *       torch_sim/neighbors/torch_nl.py:156-215 (torch_nl_linked_cell),
+ *       torch_sim/transforms.py:160-210 (wrap), 435-467 (image counts),
+ *       torch_sim/models/pair_potential.py:86-127 (full_to_half_list)
+ *   tsx_pair_forces / tsx_system_reduce
+ *       torch_sim/models/pair_potential.py:130-464 (PairPotentialModel.forward)
+ *       with the pair functions of lennard_jones.py:23-47, morse.py:23-47,
+ *       soft_sphere.py:35-59 and 90-167
+ *   tsx_md_* / tsx_md_fused_step
+ *       torch_sim/integrators/md.py:113-208, nve.py:74-130, nvt.py:27-240
+ *   tsx_fire_* / tsx_cell_*
+ *       torch_sim/optimizers/fire.py:117-499,
+ *       torch_sim/optimizers/cell_filters.py:137-178, 353-399,
+ *       torch_sim/math.py:30-224, 614-778
+ *   tsx_pair_backward_*
+ *       no reference kernel: gradients of force-matching losses (the
+ *       reference detaches dV/dr, pair_potential.py:419-425)
+ */
+#ifndef TSX_H_
+#define TSX_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TSX_OK 0
+#define TSX_ERR_ARG (-1)
+#define TSX_ERR_HIP (-2)
+
+/* device status word bits */
+#define TSX_STATUS_NL_OVERFLOW 1      /* an atom has more neighbors than kmax */
+#define TSX_STATUS_BIN_OVERFLOW 2     /* total bins exceed bin_cap            */
+#define TSX_STATUS_SHIFT_OVERFLOW 4   /* image shift outside the packed range */
+#define TSX_STATUS_SINGULAR_CELL 8    /* periodic system with singular cell   */
+#define TSX_STATUS_REBUILD 16         /* engine: list must be rebuilt         */
+
+/* pair potential kinds */
+#define TSX_LJ 0          /* 4 eps [(s/r)^12 - (s/r)^6]                  */
+#define TSX_MORSE 1       /* eps (1 - exp(-a (r - s)))^2 - eps            */
+#define TSX_SOFT_SPHERE 2 /* eps/a (1 - r/s)^a for r < s                  */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int tsx_version(void);
+
+/* Thread-local message of the last failing call (empty string if none). */
+const char* tsx_last_error(void);
+
+/* ------------------------------------------------------------------ NL --
+ * Cell-list neighbor list over concatenated systems.
+ *
+ * Per system: fractional binning with nb_k = max(1, floor(q w_k / r_list))
+ * bins along each periodic axis (w_k = distance between lattice planes,
+ * q = bins_per_rlist), one bin along non-periodic axes; stencil half-width
+ * m_k = ceil(r_list nb_k / w_k).  Atoms are binned by count (atomics),
+ * prefix-scanned, scattered into bin order and sorted by atom index within a
+ * bin (deterministic), then every atom walks its stencil and keeps the
+ * images with FP64 |wr_i - wr_j - S.L|^2 < r_list^2 on wrapped coordinates.
+ *
+ * Output is an ELL list in "thread order": row t of the list belongs to
+ * atom order[t] (atoms sorted by bin).  Entry n of row t lives at
+ *     nbr[(n / tpa) * (ld * tpa) + t * tpa + (n % tpa)]
+ * and packs the neighbor's system-local index in the low ``jbits`` bits and
+ * the unwrapped-frame image shift S' = S + w_i - w_j (valid for the input
+ * positions) as three ``sbits``-bit offset-binary fields above it.
+ * ``count[t]`` is the number of entries; overflowing rows set
+ * TSX_STATUS_NL_OVERFLOW and are truncated at kmax.
+ *
+ * half != 0 keeps one copy of every unordered pair (i < j, or i == j with a
+ * lexicographically positive S'), as full_to_half_list does.
+ *
+ * ``gate`` (device int32, may be NULL): when it points at 0 every kernel of
+ * the build returns immediately, so a captured graph can carry a rebuild that
+ * only runs when the displacement test of the previous step asked for it.
+ */
+int64_t tsx_nl_ws_bytes(int64_t n_atoms, int32_t n_systems, int32_t bin_cap);
+
+int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
+                 const uint8_t* pbc, int32_t pbc_per_system, const int32_t* sys_start,
+                 const int32_t* atom_sys, int64_t n_atoms, int32_t n_systems,
+                 double r_list, int32_t bins_per_rlist, int32_t self_interaction,
+                 int32_t half, int32_t jbits, int32_t tpa, int32_t kmax, int64_t ld,
+                 int32_t bin_cap, void* ws, uint32_t* nbr, int32_t* count,
+                 int32_t* order, int32_t* status, const int32_t* gate, void* stream);
+
+/* Exclusive scan of ``count`` into ``row_ptr`` [n+1] (int64); the total is
+ * row_ptr[n].  Used before tsx_nl_emit_pairs.  ``ws`` holds
+ * tsx_scan_ws_bytes(n) bytes. */
+int64_t tsx_scan_ws_bytes(int64_t n);
+int tsx_nl_count_pairs(const int32_t* count, int64_t n, int64_t* row_ptr, void* ws,
+                       void* stream);
+
+/* Expand the ELL list into the reference triple: mapping [2,P] int64
+ * (mapping[0] = i, mapping[1] = j), system_mapping [P] int64 and
+ * shifts [P,3] float64, in row order. */
+int tsx_nl_emit_pairs(const uint32_t* nbr, const int32_t* count, const int32_t* order,
+                      const int64_t* row_ptr, const int32_t* sys_start,
+                      const int32_t* atom_sys, int64_t n_atoms, int32_t jbits,
+                      int32_t tpa, int64_t ld, int64_t* mapping, int64_t n_pairs,
+                      int64_t* system_mapping, double* shifts, void* stream);
+
+/* Wrapped positions [N,3] and integer lattice shifts [N,3] (float64), as
+ * pbc_wrap_batched_and_get_lattice_shifts (transforms.py:160-210).  ``ws``
+ * holds tsx_wrap_ws_bytes(n_systems) bytes. */
+int64_t tsx_wrap_ws_bytes(int32_t n_systems);
+int tsx_wrap_positions(const double* pos, const double* cell_col, const uint8_t* pbc,
+                       int32_t pbc_per_system, const int32_t* atom_sys, int64_t n_atoms,
+                       int32_t n_systems, void* ws, int32_t* status, double* wrapped,
+                       double* shifts, void* stream);
+
+/* ---------------------------------------------------------------- pair --
+ * Pair energies / forces / virials over the ELL list (full or half).
+ * Pair arithmetic in FP32 on FP64 displacement vectors; the FP64 cutoff
+ * test d^2 < rc^2 matches the reference's ``distances < cutoff`` mask.
+ * Accumulation of forces, energies and virials is FP64.
+ *   params: per-species-pair table [n_species^2][3] float64 of
+ *           (sigma, epsilon, alpha); species [N] int32 (may be NULL when
+ *           n_species == 1).
+ *   forces  [N,3] float64 (written, not accumulated)
+ *   e_atom  [N]   float64 per-atom energy (written)
+ *   w_atom  [N,6] float64 per-atom virial sum_j 0.5 dr (x) f, (xx,yy,zz,yz,xz,xy)
+ *           or NULL.
+ * For a full list each ordered pair contributes to atom i only (no atomics);
+ * for a half list (half=1) the j-side contributions use FP64 atomics.
+ */
+int tsx_pair_forces(int32_t kind, const double* params, int32_t n_species,
+                    const int32_t* species, const double* pos, int32_t pos_stride,
+                    const double* cell_col, const int32_t* sys_start,
+                    const int32_t* atom_sys, const uint32_t* nbr, const int32_t* count,
+                    const int32_t* order, int64_t n_atoms, int32_t jbits, int32_t tpa,
+                    int64_t ld, double cutoff, int32_t half, double* forces,
+                    double* e_atom, double* w_atom, void* stream);
+
+/* Per-system reductions of per-atom energies and virials:
+ *   energy [S] = sum e_atom;  stress [S,3,3] = -(sum w_atom) / |det cell|;
+ *   stresses [N,3,3] (optional) = -w_atom / |det cell|.
+ * One workgroup per system, fixed summation order (deterministic). */
+int tsx_system_reduce(const double* e_atom, const double* w_atom, const double* cell_col,
+                      const int32_t* sys_start, const int32_t* atom_sys, int64_t n_atoms,
+                      int32_t n_systems, double* energy, double* stress,
+                      double* stresses, void* stream);
+
+
+/* ------------------------------------------------------------------ MD --
+ * Per-step kernels (in place, float64 [N,3] momenta / positions):
+ *   tsx_md_kick     p <- p + F dt                       (md.py:169-187)
+ *   tsx_md_drift    x <- x + (p/m) dt                   (md.py:190-208)
+ *   tsx_md_vv_half  B(dt/2) A(dt) of velocity Verlet    (nve.py:121-123)
+ *   tsx_md_baoa     B(dt/2) A(dt/2) O(dt) A(dt/2)       (nvt.py:229-233, 27-86)
+ *                   O: p <- c1 p + c2 sqrt(m) xi, c1 = exp(-gamma dt),
+ *                   c2 = sqrt(kT (1 - c1^2)); kT and gamma per system [S];
+ *                   xi = noise [N,3] when given, else Philox4x32-10 keyed by
+ *                   (seed, step, atom index).
+ */
+int tsx_md_kick(double* p, const double* f, double dt, int64_t n_atoms, void* stream);
+int tsx_md_drift(double* x, const double* p, const double* m, double dt, int64_t n_atoms,
+                 void* stream);
+int tsx_md_vv_half(double* x, double* p, const double* f, const double* m, double dt,
+                   int64_t n_atoms, void* stream);
+int tsx_md_baoa(double* x, double* p, const double* f, const double* m, const int32_t* atom_sys,
+                const double* kT, const double* gamma, double dt, const double* noise,
+                uint64_t seed, uint64_t step, int64_t n_atoms, void* stream);
+
+/* Fused step over packed [N,4] positions (x, y, z, species bits): pair forces
+ * on pos_in, the closing B(dt/2) of the current step when ``end_kick``, and,
+ * when ``do_next``, the opening half of the next step (B A O A for Langevin,
+ * B A(dt) for NVE) written to pos_out (which must differ from pos_in); else
+ * pos_out = pos_in.  Atoms farther than ``half_skin`` from ``xref`` set
+ * *flag_set = 1; *flag_clear is reset to 0.  ``step`` is a device counter
+ * giving the Philox step of the next step's noise.  w_atom may be NULL. */
+int tsx_md_fused_step(int32_t kind, const double* params, int32_t n_species,
+                      const double* pos_in, double* pos_out, const double* xref,
+                      const double* cell_col, const int32_t* sys_start,
+                      const int32_t* atom_sys, const uint32_t* nbr, const int32_t* count,
+                      const int32_t* order, int64_t n_atoms, int32_t jbits, int32_t tpa,
+                      int64_t ld, double cutoff, double* p, const double* m, double* f,
+                      double* e_atom, double* w_atom, double dt, int32_t langevin,
+                      const double* kT, const double* gamma, const double* noise,
+                      uint64_t seed, const int64_t* step, int32_t end_kick, int32_t do_next,
+                      double half_skin, int32_t* flag_set, int32_t* flag_clear,
+                      void* stream);
+
+/* Per-system energy [S] = sum of e_atom (one workgroup per system, fixed
+ * order); when ``advance``, *step += 1 afterwards. */
+int tsx_md_reduce(const double* e_atom, const int32_t* sys_start, int32_t n_systems,
+                  double* energy, int64_t* step, int32_t advance, void* stream);
+
+/* Copy n_doubles of pos into xref when *gate != 0 (or gate is NULL) and count
+ * the copies in *counter (may be NULL): the displacement reference of the
+ * Verlet skin, refreshed together with a gated list rebuild. */
+int tsx_copy_ref(const double* pos, double* xref, int64_t n_doubles, const int32_t* gate,
+                 int32_t* counter, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TSX_H_ */

@@ -1,0 +1,48 @@
+"""CPU float64 oracle for the batched classical force-and-step path.
+
+TEST INFRASTRUCTURE ONLY.  This package is a plain-PyTorch (CPU, float64)
+restatement of the TorchSim reference algorithms that the HIP product path
+replaces.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import it, and only as the checker
+(or as the timed CPU baseline).  The product package ``torch_sim_amd`` never
+imports it and has no CPU fallback.
+
+Every function cites the reference file:line it restates.  How each function
+is pinned (golden vectors from the reference tests, closed forms, scipy,
+brute force, invariants) is listed in ``PINNING`` below; a function without an
+entry there is "parity unpinned".
+
+Modules:
+    nl      -- wrap, image-count, brute-force and linked-cell neighbor lists,
+               full->half reduction (torch_sim/transforms.py,
+               torch_sim/neighbors/torch_nl.py, models/pair_potential.py).
+    models  -- LJ / Morse / soft-sphere pair functions and the half-list
+               pair-potential forward with autograd forces and virial stress
+               (torch_sim/models/*.py).
+    md      -- Maxwell-Boltzmann momenta, velocity Verlet, BAOAB Langevin
+               (torch_sim/integrators/{md,nve,nvt}.py).
+    fire    -- ASE-FIRE / VV-FIRE and the Frechet cell filter
+               (torch_sim/optimizers/{fire,cell_filters}.py).
+    mat33   -- 3x3 expm / logm / Frechet derivative (torch_sim/math.py),
+               checked against scipy.linalg.
+    units   -- MetalUnits constants (torch_sim/units.py).
+"""
+
+PINNING = {
+    "nl.wrap_positions": "reference KATs test_transforms.py:358-416 (golden fixture)",
+    "nl.num_cell_repeats": "reference KATs test_transforms.py:993-1025; closed form",
+    "nl.brute_force_nl": "exhaustive enumeration; closed-form fcc shell counts "
+    "(Ar 48 pairs, test_transforms.py:1334-1368; 78-neighbour shells); "
+    "lattice-image invariance (test_neighbors.py:232-304)",
+    "nl.linked_cell_nl": "set-equal to brute_force_nl on random/triclinic inputs",
+    "nl.full_to_half_list": "reference KAT test_pair_potential.py:102-124",
+    "models.*_pair": "closed forms test_lennard_jones.py / test_morse.py / "
+    "test_soft_sphere.py; finite differences",
+    "models.pair_forward": "perfect-lattice closed sums (E/atom, F=0); F=-dE/dx and "
+    "stress=-(1/V)dE/deps by finite differences; half==full; sum F=0",
+    "md.*": "BAOAB one-step closed form with supplied noise; NVE energy drift; "
+    "COM removal; batch==single",
+    "fire.*": "P<0 branch closed form (test_optimizers.py:600-662); convergence of "
+    "perfect lattices; Frechet adjoint identity vs 9-direction sum",
+    "mat33.*": "scipy.linalg.expm / logm / expm_frechet",
+}

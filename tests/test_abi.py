@@ -1,0 +1,37 @@
+"""The C-ABI library loads and exports every entry point declared in
+include/tsx.h (no device work)."""
+
+import os
+import re
+
+import torch_sim_amd as tsa
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(__file__)), "include", "tsx.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^(?:int|int64_t|const char\*)\s+(tsx_\w+)\(", text, re.M)))
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    assert "tsx_nl_build" in names and "tsx_pair_forces" in names and len(names) >= 15
+
+
+def test_library_exports_every_declared_symbol():
+    lib = tsa._lib.lib()
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_table_covers_header():
+    assert set(declared()) <= set(tsa._lib._SIGS), set(declared()) - set(tsa._lib._SIGS)
+
+
+def test_host_side_queries():
+    lib = tsa._lib.lib()
+    assert lib.tsx_version() >= 100
+    assert lib.tsx_nl_ws_bytes(1000, 4, 100) > 1000 * 4 * 8
+    assert lib.tsx_scan_ws_bytes(10**6) >= 8 * 2 * (10**6 // 2048)
+    assert lib.tsx_last_error() is not None

@@ -1,0 +1,145 @@
+"""GPU parity of the HIP cell-list neighbor list against the FP64 oracle.
+
+Pair sets must be identical after sorting (bit-exact integer triples)."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import torch_sim_amd as tsa
+from torch_sim_amd.neighbors import NeighborList, SystemIndex
+from oracle import nl as onl
+
+pytestmark = pytest.mark.gpu
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
+F64 = torch.float64
+
+
+def _gpu_nl(pos, cell_row, pbc, cutoff, idx, si=False, dev="cuda"):
+    out = tsa.torchsim_nl(pos.to(dev), cell_row.to(dev), pbc.to(dev), cutoff, idx.to(dev), si)
+    return onl.sort_pairs(*(t.cpu() for t in out))
+
+
+def _assert_same(a, b):
+    assert a[0].shape == b[0].shape, (a[0].shape, b[0].shape)
+    assert torch.equal(a[0], b[0])
+    assert torch.equal(a[1], b[1])
+    assert torch.equal(a[2].to(F64), b[2].to(F64))
+
+
+def _cacrp2o7():
+    g = GOLD["cacrp2o7"]
+    pos = torch.tensor(g["positions"], dtype=F64)
+    return pos, torch.tensor(g["cell_row"], dtype=F64).view(1, 3, 3), torch.tensor([[True] * 3]), torch.zeros(len(pos), dtype=torch.long)
+
+
+@pytest.mark.parametrize("cutoff", [2.0, 5.0, 7.0, 11.0])
+@pytest.mark.parametrize("self_interaction", [False, True])
+def test_triclinic_pair_set_exact(dev, cutoff, self_interaction):
+    pos, cell, pbc, idx = _cacrp2o7()
+    ref = onl.brute_force_nl(pos, cell, pbc, cutoff, idx, self_interaction)
+    _assert_same(_gpu_nl(pos, cell, pbc, cutoff, idx, self_interaction), ref)
+
+
+@pytest.mark.parametrize("shift_mode", ["uniform", "per_atom"])
+def test_unwrapped_input_pair_set_exact(dev, shift_mode):
+    pos, cell, pbc, idx = _cacrp2o7()
+    n = pos.shape[0]
+    if shift_mode == "uniform":
+        ints = torch.tensor(GOLD["lattice_shift_triplets"]["uniform"][1], dtype=F64).expand(n, 3)
+    else:
+        ar = torch.arange(n)
+        ints = torch.stack([(ar % 3) - 1, (ar % 5) - 2, (ar % 4) - 2], 1).to(F64)
+    shifted = pos + ints @ cell[0]
+    ref = onl.brute_force_nl(shifted, cell, pbc, 5.0, idx)
+    _assert_same(_gpu_nl(shifted, cell, pbc, 5.0, idx), ref)
+
+
+def test_batched_uneven_triclinic_exact(dev):
+    st = tsa.workloads.soft_sphere_batch(6, n_min=16, n_max=96, device="cpu", seed=4)
+    cell_row = st.cell.mT.contiguous()
+    for cutoff in (1.4, 2.3):
+        ref = onl.brute_force_nl(st.positions, cell_row, st.pbc, cutoff, st.system_idx)
+        _assert_same(_gpu_nl(st.positions, cell_row, st.pbc, cutoff, st.system_idx), ref)
+
+
+def test_small_cells_many_images_exact(dev):
+    # Cu 2x2x2 (7.22 A) and 1x1x1 (3.61 A) cells with rc = 6 A: each pair appears
+    # under several image shifts
+    st = tsa.workloads.cu_batch(3, device="cpu", reps=[[2, 2, 2], [1, 1, 1], [1, 2, 3]], seed=2)
+    ref = onl.brute_force_nl(st.positions, st.cell.mT, st.pbc, 6.0, st.system_idx)
+    _assert_same(_gpu_nl(st.positions, st.cell.mT.contiguous(), st.pbc, 6.0, st.system_idx), ref)
+
+
+def test_per_system_pbc_exact(dev):
+    st = tsa.workloads.soft_sphere_batch(3, n_min=20, n_max=40, device="cpu", seed=7)
+    pbc = torch.tensor([[True, True, True], [True, False, True], [False, False, False]])
+    ref = onl.brute_force_nl(st.positions, st.cell.mT, pbc, 1.8, st.system_idx)
+    _assert_same(_gpu_nl(st.positions, st.cell.mT.contiguous(), pbc, 1.8, st.system_idx), ref)
+
+
+def test_edge_cases_and_empty(dev):
+    g = GOLD["nl_edge_case"]
+    for p in (True, False):
+        for si in (True, False):
+            pos = torch.tensor(g["positions"], dtype=F64)
+            cell = torch.eye(3, dtype=F64) * g["cell_diag"]
+            out = tsa.torchsim_nl(pos.cuda(), cell.cuda(), torch.tensor([p] * 3).cuda(), g["cutoff"],
+                                  torch.zeros(2, dtype=torch.long).cuda(), si)
+            assert out[0].shape[1] > 0 and (out[1] == 0).all()
+    g = GOLD["nl_empty"]
+    out = tsa.torchsim_nl(torch.tensor(g["positions"], dtype=F64).cuda(), (torch.eye(3, dtype=F64) * g["cell_diag"]).cuda(),
+                          torch.tensor(g["pbc"]).cuda(), g["cutoff"], torch.zeros(2, dtype=torch.long).cuda())
+    assert out[0].shape == (2, 0) and out[1].numel() == 0 and out[2].shape == (0, 3)
+
+
+def test_half_list_matches_full_to_half(dev):
+    pos, cell, pbc, idx = _cacrp2o7()
+    full = onl.brute_force_nl(pos, cell, pbc, 7.0, idx)
+    ref = onl.sort_pairs(*onl.full_to_half_list(*full))
+    si = SystemIndex(idx.cuda(), 1)
+    nl = NeighborList(7.0, half=True, tpa=2)
+    nl.build(pos.cuda(), cell.transpose(1, 2).contiguous().cuda(), pbc.cuda(), si)
+    got = onl.sort_pairs(*(t.cpu() for t in nl.pairs(si, F64)))
+    _assert_same(got, ref)
+
+
+@pytest.mark.parametrize("tpa,q", [(1, 1), (4, 1), (8, 2)])
+def test_layouts_and_bin_sizes_give_same_set(dev, tpa, q):
+    st = tsa.workloads.cu_batch(4, device="cpu", seed=5)
+    ref = onl.brute_force_nl(st.positions, st.cell.mT, st.pbc, 6.0, st.system_idx)
+    si = SystemIndex(st.system_idx.cuda(), st.n_systems)
+    nl = NeighborList(6.0, tpa=tpa, bins_per_rlist=q)
+    nl.build(st.positions.cuda(), st.cell.cuda().contiguous(), st.pbc.cuda(), si)
+    _assert_same(onl.sort_pairs(*(t.cpu() for t in nl.pairs(si, F64))), ref)
+
+
+def test_wrap_positions_matches_oracle(dev):
+    g = GOLD["wrap_batched"]
+    cell = torch.tensor(g["cell_row"], dtype=F64)
+    pos = torch.tensor(g["positions"], dtype=F64)
+    idx = torch.tensor(g["system_idx"])
+    pbc = torch.tensor(g["pbc"])
+    w, s = tsa.neighbors.wrap_positions(pos.cuda(), cell.cuda(), idx.cuda(), pbc.cuda())
+    rw, rs = onl.wrap_positions(pos, cell, idx, pbc)
+    torch.testing.assert_close(w.cpu(), rw, atol=1e-12, rtol=0)
+    assert torch.equal(s.cpu(), rs)
+    g = GOLD["wrap_non_periodic_axis"]
+    w, s = tsa.neighbors.wrap_positions(torch.tensor(g["positions"], dtype=F64).cuda(),
+                                        torch.tensor(g["cell_row"], dtype=F64).cuda(),
+                                        torch.zeros(1, dtype=torch.long).cuda(), torch.tensor(g["pbc"]).cuda())
+    assert w.cpu().tolist() == g["expected_wrapped"] and s.cpu().tolist() == g["expected_shifts"]
+
+
+def test_overflow_retry_resizes(dev):
+    st = tsa.workloads.cu_batch(2, device="cpu", seed=8)
+    si = SystemIndex(st.system_idx.cuda(), st.n_systems)
+    nl = NeighborList(6.0, tpa=1)
+    nl.ensure(st.n_atoms, st.n_systems, 8, 2, 11, torch.device("cuda"))
+    nl.build(st.positions.cuda(), st.cell.cuda().contiguous(), st.pbc.cuda(), si)
+    assert nl.kmax >= 78 and int(nl.count.max()) >= 78
+    ref = onl.brute_force_nl(st.positions, st.cell.mT, st.pbc, 6.0, st.system_idx)
+    _assert_same(onl.sort_pairs(*(t.cpu() for t in nl.pairs(si, F64))), ref)

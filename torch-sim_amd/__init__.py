@@ -1,0 +1,39 @@
+"""torch_sim_amd -- MI355X-native batched classical force-and-step path.
+
+A drop-in for TorchSim's classical pair-potential path: the periodic
+cell-list neighbor list over concatenated systems, the Lennard-Jones / Morse /
+soft-sphere energy / force / virial models behind ``model(state)``, and the
+``integrate`` / ``optimize`` entry points, all running hand-written gfx950 HIP
+kernels through the C ABI in ``include/tsx.h``.
+"""
+
+from . import _lib, neighbors, units, workloads
+from .engine import BatchedMD
+from .integrators import (
+    INTEGRATOR_REGISTRY,
+    Integrator,
+    MDState,
+    initialize_momenta,
+    momentum_step,
+    nve_init,
+    nve_step,
+    nvt_langevin_init,
+    nvt_langevin_step,
+    position_step,
+)
+from .models import (
+    LennardJonesModel,
+    ModelInterface,
+    MorseModel,
+    MultiLennardJonesModel,
+    PairPotentialModel,
+    SoftSphereModel,
+    SoftSphereMultiModel,
+)
+from .neighbors import torchsim_nl
+from .quantities import calc_kinetic_energy, calc_kT, system_wise_max_force
+from .runners import integrate
+from .state import SimState, concatenate_states, initialize_state
+from .units import MetalUnits
+
+__version__ = "0.1.0"

@@ -1,0 +1,677 @@
+// Cell-list neighbor list over concatenated periodic systems (gfx950).
+//
+// Replaces the reference's padded, host-synchronising linked-cell search
+// (torch_sim/transforms.py:1009-1373, torch_sim/neighbors/torch_nl.py:156-215)
+// with per-system fractional grids:
+//   setup   : one workgroup computes each system's inverse cell, plane
+//             spacings, grid size nb_k and stencil half-width m_k, and a scan
+//             of bin counts into per-system bin offsets.
+//   bin     : one thread per atom, fractional coordinate -> bin, atomic count.
+//   scan    : exclusive scan of the bin counts (wave64 shuffles + LDS).
+//   scatter : atoms -> bin order; one-bin systems are written in atom order.
+//   sort    : each multi-atom bin is sorted by atom index (deterministic).
+//   wrap    : wrapped FP64 coordinates (transforms.py:160-210) in bin order.
+//   fill    : one thread per atom walks its stencil, FP64 distance test on
+//             wrapped coordinates (torch_nl.py:87), unwrapped-frame shift
+//             (torch_nl.py:214), ELL write with overflow detection.
+#include "tsx_common.h"
+
+#include <cstdarg>
+
+namespace tsx {
+
+static thread_local char g_err[512] = {0};
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+// Per-system geometry record (doubles):
+//   [0..8]  row cell L (row k = lattice vector k)
+//   [9..17] Linv with f = r . Linv  (f_k = sum_m r_m Linv[m][k])
+//   [18..20] plane spacings w_k,  [21] |det|,  [22] active (1.0/0.0)
+constexpr int GEOM = 24;
+// Per-system grid record (int32): nb[3], m[3], bin_off, flags (bit k = pbc_k)
+constexpr int GRID = 8;
+constexpr int META = 16;  // [0] total bins
+
+struct NLWs {
+  double* geom;
+  int32_t* grid;
+  int32_t* meta;
+  int32_t* bin_count;  // bin_cap + 1
+  int32_t* bin_start;  // bin_cap + 1
+  int32_t* atom_bin;   // N
+  int32_t* slot;       // N
+  double* wr;          // N x 4 (wrapped xyz, packed wrap shift)
+  int32_t* scan_tmp;   // scan block sums
+};
+
+static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static size_t ws_layout(int64_t n, int32_t s, int32_t bin_cap, NLWs* w, char* base) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char* p = base ? base + off : nullptr;
+    off += align256(bytes);
+    return p;
+  };
+  const int64_t nb = (int64_t)bin_cap + 1;
+  void* p_geom = take(sizeof(double) * GEOM * (size_t)(s > 0 ? s : 1));
+  void* p_grid = take(sizeof(int32_t) * GRID * (size_t)(s > 0 ? s : 1));
+  void* p_meta = take(sizeof(int32_t) * META);
+  void* p_bc = take(sizeof(int32_t) * nb);
+  void* p_bs = take(sizeof(int32_t) * nb);
+  void* p_ab = take(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  void* p_sl = take(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  void* p_wr = take(sizeof(double) * 4 * (size_t)(n > 0 ? n : 1));
+  void* p_st = take(sizeof(int32_t) * 4096);
+  if (w) {
+    w->geom = (double*)p_geom;
+    w->grid = (int32_t*)p_grid;
+    w->meta = (int32_t*)p_meta;
+    w->bin_count = (int32_t*)p_bc;
+    w->bin_start = (int32_t*)p_bs;
+    w->atom_bin = (int32_t*)p_ab;
+    w->slot = (int32_t*)p_sl;
+    w->wr = (double*)p_wr;
+    w->scan_tmp = (int32_t*)p_st;
+  }
+  return off;
+}
+
+// Pack a wrap shift (three ints in [-2^20, 2^20)) into 64 bits.
+__device__ __forceinline__ double pack_w(int a, int b, int c) {
+  const uint64_t m = (1ull << 21) - 1;
+  const uint64_t v = ((uint64_t)(a + (1 << 20)) & m) | (((uint64_t)(b + (1 << 20)) & m) << 21) |
+                     (((uint64_t)(c + (1 << 20)) & m) << 42);
+  return __longlong_as_double((long long)v);
+}
+__device__ __forceinline__ void unpack_w(double d, int* w) {
+  const uint64_t v = (uint64_t)__double_as_longlong(d);
+  const uint64_t m = (1ull << 21) - 1;
+  w[0] = (int)(v & m) - (1 << 20);
+  w[1] = (int)((v >> 21) & m) - (1 << 20);
+  w[2] = (int)((v >> 42) & m) - (1 << 20);
+}
+
+__device__ __forceinline__ bool pbc_of(const uint8_t* pbc, int per_sys, int s, int k) {
+  return pbc[per_sys ? 3 * s + k : k] != 0;
+}
+
+// ------------------------------------------------------------------ setup
+__global__ void k_setup(const double* __restrict__ cell_col, const uint8_t* __restrict__ pbc,
+                        int pbc_per_sys, int n_sys, double r_list, int q, int bin_cap,
+                        double* __restrict__ geom, int32_t* __restrict__ grid,
+                        int32_t* __restrict__ meta, int32_t* __restrict__ status,
+                        int32_t* __restrict__ bin_count, const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  // every block zeroes part of the bin counters
+  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b <= bin_cap;
+       b += (int64_t)gridDim.x * blockDim.x)
+    bin_count[b] = 0;
+  if (blockIdx.x != 0) return;
+  __shared__ int32_t sh_nb[1024];
+  __shared__ int32_t sh_carry;
+  if (threadIdx.x == 0) sh_carry = 0;
+  __syncthreads();
+  for (int base = 0; base < n_sys; base += blockDim.x) {
+    const int s = base + threadIdx.x;
+    int nbins = 0;
+    if (s < n_sys) {
+      double L[9], inv[9];
+      for (int k = 0; k < 3; ++k)
+        for (int m = 0; m < 3; ++m) L[3 * k + m] = cell_col[9 * s + 3 * m + k];
+      const double det = det33(L);
+      const bool any_pbc =
+          pbc_of(pbc, pbc_per_sys, s, 0) || pbc_of(pbc, pbc_per_sys, s, 1) ||
+          pbc_of(pbc, pbc_per_sys, s, 2);
+      const bool invertible = isfinite(det) && fabs(det) > 1e-12;
+      const bool active = any_pbc && invertible;
+      if (any_pbc && !invertible) atomicOr(status, TSX_STATUS_SINGULAR_CELL);
+      // f = r . Linv  with Linv = inverse of L (row convention)
+      if (invertible) {
+        inv33(L, inv);
+      } else {
+        for (int k = 0; k < 9; ++k) inv[k] = 0.0;
+      }
+      double* g = geom + (size_t)GEOM * s;
+      for (int k = 0; k < 9; ++k) g[k] = L[k];
+      for (int k = 0; k < 9; ++k) g[9 + k] = inv[k];
+      int nb[3], mm[3], flags = 0;
+      for (int k = 0; k < 3; ++k) {
+        // plane spacing along k = 1 / |column k of Linv|
+        const double c2 = inv[k] * inv[k] + inv[3 + k] * inv[3 + k] + inv[6 + k] * inv[6 + k];
+        const double w = invertible ? 1.0 / sqrt(c2) : 0.0;
+        g[18 + k] = w;
+        const bool p = active && pbc_of(pbc, pbc_per_sys, s, k);
+        if (p) {
+          flags |= 1 << k;
+          double nbf = floor((double)q * w / r_list);
+          if (nbf < 1.0) nbf = 1.0;
+          if (nbf > 1024.0) nbf = 1024.0;
+          nb[k] = (int)nbf;
+          mm[k] = (int)ceil(r_list * nb[k] / w);
+        } else {
+          nb[k] = 1;
+          mm[k] = 0;
+        }
+      }
+      g[21] = fabs(det);
+      g[22] = active ? 1.0 : 0.0;
+      g[23] = 0.0;
+      int32_t* gr = grid + (size_t)GRID * s;
+      for (int k = 0; k < 3; ++k) {
+        gr[k] = nb[k];
+        gr[3 + k] = mm[k];
+      }
+      gr[7] = flags;
+      nbins = nb[0] * nb[1] * nb[2];
+    }
+    // block exclusive scan of nbins
+    sh_nb[threadIdx.x] = nbins;
+    __syncthreads();
+    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+      int v = threadIdx.x >= (unsigned)off ? sh_nb[threadIdx.x - off] : 0;
+      __syncthreads();
+      sh_nb[threadIdx.x] += v;
+      __syncthreads();
+    }
+    const int incl = sh_nb[threadIdx.x];
+    if (s < n_sys) grid[(size_t)GRID * s + 6] = sh_carry + incl - nbins;
+    __syncthreads();
+    if (threadIdx.x == blockDim.x - 1) sh_carry += incl;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    meta[0] = sh_carry;
+    if (sh_carry > bin_cap) atomicOr(status, TSX_STATUS_BIN_OVERFLOW);
+  }
+}
+
+// fractional coordinates, wrap shift and wrapped position of one atom
+struct WrapOut {
+  double f[3];  // wrapped fractional coordinates
+  int w[3];     // integer wrap shift
+  double wr[3]; // wrapped cartesian position
+};
+
+__device__ __forceinline__ WrapOut wrap_atom(const double* __restrict__ r, const double* g,
+                                             int flags) {
+  WrapOut o;
+  const bool active = g[22] != 0.0;
+  if (!active) {
+    for (int k = 0; k < 3; ++k) {
+      o.f[k] = 0.0;
+      o.w[k] = 0;
+      o.wr[k] = r[k];
+    }
+    return o;
+  }
+  double f[3];
+  for (int k = 0; k < 3; ++k) f[k] = r[0] * g[9 + k] + r[1] * g[12 + k] + r[2] * g[15 + k];
+  for (int k = 0; k < 3; ++k) {
+    double fl = 0.0;
+    if (flags & (1 << k)) fl = floor(f[k]);
+    o.w[k] = (int)fl;
+    o.f[k] = f[k] - fl;
+  }
+  for (int m = 0; m < 3; ++m)
+    o.wr[m] = g[m] * o.f[0] + g[3 + m] * o.f[1] + g[6 + m] * o.f[2];
+  return o;
+}
+
+__device__ __forceinline__ int bin_of(const WrapOut& o, const int32_t* gr) {
+  int b[3];
+  for (int k = 0; k < 3; ++k) {
+    if (gr[7] & (1 << k)) {
+      int v = (int)floor(o.f[k] * gr[k]);
+      v = v < 0 ? 0 : (v >= gr[k] ? gr[k] - 1 : v);
+      b[k] = v;
+    } else {
+      b[k] = 0;
+    }
+  }
+  return gr[6] + (b[0] * gr[1] + b[1]) * gr[2] + b[2];
+}
+
+__global__ void k_bin(const double* __restrict__ pos, int ps, const int32_t* __restrict__ atom_sys,
+                      int64_t n, const double* __restrict__ geom, const int32_t* __restrict__ grid,
+                      const int32_t* __restrict__ meta, int bin_cap,
+                      int32_t* __restrict__ bin_count, int32_t* __restrict__ atom_bin,
+                      int32_t* __restrict__ slot, const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (meta[0] > bin_cap) return;
+  const int s = atom_sys[i];
+  const int32_t* gr = grid + (size_t)GRID * s;
+  const WrapOut o = wrap_atom(pos + ps * i, geom + (size_t)GEOM * s, gr[7]);
+  const int b = bin_of(o, gr);
+  atom_bin[i] = b;
+  slot[i] = atomicAdd(&bin_count[b], 1);
+}
+
+// Exclusive scan of int32 -> T over [0, n] (one extra element = total),
+// single workgroup of 1024 threads; each thread owns a contiguous chunk.
+template <typename Tin, typename T>
+__global__ void k_scan_1block(const Tin* __restrict__ in, int64_t n, T* __restrict__ out, const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  __shared__ T sh[1024];
+  const int64_t chunk = (n + blockDim.x - 1) / blockDim.x;
+  const int64_t lo = (int64_t)threadIdx.x * chunk;
+  const int64_t hi = lo + chunk < n ? lo + chunk : n;
+  T s = 0;
+  for (int64_t k = lo; k < hi; ++k) s += (T)in[k];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+    T v = threadIdx.x >= (unsigned)off ? sh[threadIdx.x - off] : (T)0;
+    __syncthreads();
+    sh[threadIdx.x] += v;
+    __syncthreads();
+  }
+  T run = sh[threadIdx.x] - s;
+  for (int64_t k = lo; k < hi; ++k) {
+    out[k] = run;
+    run += (T)in[k];
+  }
+  if (threadIdx.x == blockDim.x - 1) out[n] = sh[threadIdx.x];
+}
+
+// Multi-block exclusive scan for large n: per-block sums, scan of the sums,
+// then per-block scan with the carried offset.
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_TILE = SCAN_ITEMS * SCAN_THREADS;
+
+__global__ void k_scan_partials(const int32_t* __restrict__ in, int64_t n,
+                                int64_t* __restrict__ part) {
+  __shared__ double sh[SCAN_THREADS / 64];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  int64_t s = 0;
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const int64_t idx = base + (int64_t)k * SCAN_THREADS + threadIdx.x;
+    if (idx < n) s += in[idx];
+  }
+  const double t = block_sum((double)s, sh);  // exact for counts < 2^53
+  if (threadIdx.x == 0) part[blockIdx.x] = (int64_t)t;
+}
+
+__global__ void k_scan_final(const int32_t* __restrict__ in, int64_t n,
+                             const int64_t* __restrict__ part_scan, int64_t* __restrict__ out) {
+  __shared__ int64_t sh[SCAN_THREADS];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  const int64_t lo = base + (int64_t)threadIdx.x * SCAN_ITEMS;
+  int64_t v[SCAN_ITEMS];
+  int64_t s = 0;
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const int64_t idx = lo + k;
+    v[k] = idx < n ? in[idx] : 0;
+    s += v[k];
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < SCAN_THREADS; off <<= 1) {
+    int64_t t = threadIdx.x >= (unsigned)off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int64_t run = part_scan[blockIdx.x] + sh[threadIdx.x] - s;
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const int64_t idx = lo + k;
+    if (idx < n) out[idx] = run;
+    run += v[k];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_THREADS - 1) out[n] = run;
+}
+
+__global__ void k_scatter(int64_t n, const int32_t* __restrict__ atom_sys,
+                          const int32_t* __restrict__ sys_start, const int32_t* __restrict__ grid,
+                          const int32_t* __restrict__ meta, int bin_cap,
+                          const int32_t* __restrict__ atom_bin, const int32_t* __restrict__ slot,
+                          const int32_t* __restrict__ bin_start, int32_t* __restrict__ order, const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (meta[0] > bin_cap) return;
+  const int s = atom_sys[i];
+  const int32_t* gr = grid + (size_t)GRID * s;
+  const int single = gr[0] * gr[1] * gr[2] == 1;
+  // one-bin systems keep atom order; otherwise the atomic slot, sorted later
+  const int64_t dst = single ? (int64_t)i : (int64_t)bin_start[atom_bin[i]] + slot[i];
+  (void)sys_start;
+  order[dst] = (int32_t)i;
+}
+
+__global__ void k_sort_bins(const int32_t* __restrict__ meta, int bin_cap,
+                            const int32_t* __restrict__ bin_start, int32_t* __restrict__ order, const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int total = meta[0];
+  if (total > bin_cap || b >= total) return;
+  const int lo = bin_start[b], hi = bin_start[b + 1];
+  for (int k = lo + 1; k < hi; ++k) {
+    const int v = order[k];
+    int p = k - 1;
+    while (p >= lo && order[p] > v) {
+      order[p + 1] = order[p];
+      --p;
+    }
+    order[p + 1] = v;
+  }
+}
+
+__global__ void k_wrap_sorted(const double* __restrict__ pos, int ps,
+                              const int32_t* __restrict__ atom_sys, int64_t n,
+                              const double* __restrict__ geom, const int32_t* __restrict__ grid,
+                              const int32_t* __restrict__ meta, int bin_cap,
+                              const int32_t* __restrict__ order, double* __restrict__ wr, const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  if (meta[0] > bin_cap) return;
+  const int i = order[t];
+  const int s = atom_sys[i];
+  const WrapOut o = wrap_atom(pos + (int64_t)ps * i, geom + (size_t)GEOM * s,
+                              grid[(size_t)GRID * s + 7]);
+  double4 v;
+  v.x = o.wr[0];
+  v.y = o.wr[1];
+  v.z = o.wr[2];
+  v.w = pack_w(o.w[0], o.w[1], o.w[2]);
+  reinterpret_cast<double4*>(wr)[t] = v;
+}
+
+__device__ __forceinline__ int floordiv(int a, int b) {
+  int q = a / b;
+  if ((a % b != 0) && ((a < 0) != (b < 0))) --q;
+  return q;
+}
+
+// squared distance |a - b - c|^2 in the reference operation order, no FMA
+#pragma clang fp contract(off)
+__device__ __forceinline__ double dist2_ref(double ax, double ay, double az, double bx,
+                                            double by, double bz, double cx, double cy,
+                                            double cz) {
+  const double dx = (ax - bx) - cx;
+  const double dy = (ay - by) - cy;
+  const double dz = (az - bz) - cz;
+  return (dx * dx + dy * dy) + dz * dz;
+}
+#pragma clang fp contract(on)
+
+__device__ __forceinline__ int64_t ell_index(int64_t t, int n, int tpa, int64_t ld) {
+  return (int64_t)(n / tpa) * (ld * tpa) + t * tpa + (n % tpa);
+}
+
+__global__ void k_fill(const int32_t* __restrict__ atom_sys, const int32_t* __restrict__ sys_start,
+                       int64_t n_atoms, const double* __restrict__ geom,
+                       const int32_t* __restrict__ grid, const int32_t* __restrict__ meta,
+                       int bin_cap, const int32_t* __restrict__ bin_start,
+                       const int32_t* __restrict__ atom_bin,
+                       const int32_t* __restrict__ order, const double* __restrict__ wr,
+                       double rl2, int self_int, int half, int jbits, int tpa, int kmax,
+                       int64_t ld, uint32_t* __restrict__ nbr, int32_t* __restrict__ count,
+                       int32_t* __restrict__ status, const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_atoms) return;
+  if (meta[0] > bin_cap) {
+    count[t] = 0;
+    return;
+  }
+  const int i = order[t];
+  const int s = atom_sys[i];
+  const int base = sys_start[s];
+  const double* g = geom + (size_t)GEOM * s;
+  const int32_t* gr = grid + (size_t)GRID * s;
+  const double4 wi = reinterpret_cast<const double4*>(wr)[t];
+  int w_i[3];
+  unpack_w(wi.w, w_i);
+  int bi[3];
+  {
+    const int local = atom_bin[i] - gr[6];
+    bi[2] = local % gr[2];
+    bi[1] = (local / gr[2]) % gr[1];
+    bi[0] = local / (gr[1] * gr[2]);
+  }
+  const int sbits = (32 - jbits) / 3;
+  const int bias = 1 << (sbits - 1);
+  const uint32_t fmask = (1u << sbits) - 1u;
+  int cnt = 0;
+  bool shift_ovf = false;
+  for (int dx = -gr[3]; dx <= gr[3]; ++dx) {
+    const int cx = bi[0] + dx;
+    const int sx = floordiv(cx, gr[0]);
+    const int bx = cx - sx * gr[0];
+    for (int dy = -gr[4]; dy <= gr[4]; ++dy) {
+      const int cy = bi[1] + dy;
+      const int sy = floordiv(cy, gr[1]);
+      const int by = cy - sy * gr[1];
+      for (int dz = -gr[5]; dz <= gr[5]; ++dz) {
+        const int cz = bi[2] + dz;
+        const int sz = floordiv(cz, gr[2]);
+        const int bz = cz - sz * gr[2];
+        const int b = gr[6] + (bx * gr[1] + by) * gr[2] + bz;
+        // S . L in the reference einsum order
+        const double csx = (double)sx * g[0] + (double)sy * g[3] + (double)sz * g[6];
+        const double csy = (double)sx * g[1] + (double)sy * g[4] + (double)sz * g[7];
+        const double csz = (double)sx * g[2] + (double)sy * g[5] + (double)sz * g[8];
+        const bool zero_shift = (sx | sy | sz) == 0;
+        const int k0 = bin_start[b], k1 = bin_start[b + 1];
+        for (int k = k0; k < k1; ++k) {
+          const double4 wj = reinterpret_cast<const double4*>(wr)[k];
+          const double d2 = dist2_ref(wi.x, wi.y, wi.z, wj.x, wj.y, wj.z, csx, csy, csz);
+          if (!(d2 < rl2)) continue;
+          const int j = order[k];
+          if (j == i && zero_shift && !self_int) continue;
+          int w_j[3];
+          unpack_w(wj.w, w_j);
+          const int ux = sx + w_i[0] - w_j[0];
+          const int uy = sy + w_i[1] - w_j[1];
+          const int uz = sz + w_i[2] - w_j[2];
+          if (half) {
+            bool keep;
+            if (i != j) {
+              keep = i < j;
+            } else {
+              const int first = ux != 0 ? ux : (uy != 0 ? uy : uz);
+              keep = first > 0;
+            }
+            if (!keep) continue;
+          }
+          if (ux < -bias || ux >= bias || uy < -bias || uy >= bias || uz < -bias || uz >= bias) {
+            shift_ovf = true;
+            continue;
+          }
+          const uint32_t word = (uint32_t)(j - base) |
+                                ((((uint32_t)(ux + bias) & fmask) |
+                                  (((uint32_t)(uy + bias) & fmask) << sbits) |
+                                  (((uint32_t)(uz + bias) & fmask) << (2 * sbits)))
+                                 << jbits);
+          if (cnt < kmax) nbr[ell_index(t, cnt, tpa, ld)] = word;
+          ++cnt;
+        }
+      }
+    }
+  }
+  if (cnt > kmax) {
+    atomicOr(status, TSX_STATUS_NL_OVERFLOW);
+    cnt = kmax;
+  }
+  if (shift_ovf) atomicOr(status, TSX_STATUS_SHIFT_OVERFLOW);
+  count[t] = cnt;
+}
+
+__global__ void k_emit(const uint32_t* __restrict__ nbr, const int32_t* __restrict__ count,
+                       const int32_t* __restrict__ order, const int64_t* __restrict__ row_ptr,
+                       const int32_t* __restrict__ sys_start, const int32_t* __restrict__ atom_sys,
+                       int64_t n_atoms, int jbits, int tpa, int64_t ld, int64_t* __restrict__ map,
+                       int64_t n_pairs, int64_t* __restrict__ sysmap, double* __restrict__ shifts) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_atoms) return;
+  const int i = order[t];
+  const int s = atom_sys[i];
+  const int base = sys_start[s];
+  const int sbits = (32 - jbits) / 3;
+  const int bias = 1 << (sbits - 1);
+  const uint32_t fmask = (1u << sbits) - 1u;
+  const uint32_t jmask = (jbits >= 32) ? 0xffffffffu : ((1u << jbits) - 1u);
+  int64_t p = row_ptr[t];
+  const int c = count[t];
+  for (int n = 0; n < c; ++n, ++p) {
+    const uint32_t w = nbr[ell_index(t, n, tpa, ld)];
+    const uint32_t sf = w >> jbits;
+    map[p] = i;
+    map[n_pairs + p] = base + (int)(w & jmask);
+    sysmap[p] = s;
+    shifts[3 * p + 0] = (double)((int)(sf & fmask) - bias);
+    shifts[3 * p + 1] = (double)((int)((sf >> sbits) & fmask) - bias);
+    shifts[3 * p + 2] = (double)((int)((sf >> (2 * sbits)) & fmask) - bias);
+  }
+}
+
+__global__ void k_wrap_out(const double* __restrict__ pos, const double* __restrict__ geom,
+                           const int32_t* __restrict__ grid, const int32_t* __restrict__ atom_sys,
+                           int64_t n, double* __restrict__ wrapped, double* __restrict__ shifts) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = atom_sys[i];
+  const WrapOut o =
+      wrap_atom(pos + 3 * i, geom + (size_t)GEOM * s, grid[(size_t)GRID * s + 7]);
+  for (int k = 0; k < 3; ++k) {
+    wrapped[3 * i + k] = o.wr[k];
+    shifts[3 * i + k] = (double)o.w[k];
+  }
+}
+
+}  // namespace tsx
+
+using namespace tsx;
+
+extern "C" {
+
+int tsx_version(void) { return 100; }
+
+const char* tsx_last_error(void) { return tsx::g_err; }
+
+int64_t tsx_nl_ws_bytes(int64_t n_atoms, int32_t n_systems, int32_t bin_cap) {
+  return (int64_t)ws_layout(n_atoms, n_systems, bin_cap, nullptr, nullptr);
+}
+
+int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
+                 const uint8_t* pbc, int32_t pbc_per_system, const int32_t* sys_start,
+                 const int32_t* atom_sys, int64_t n_atoms, int32_t n_systems, double r_list,
+                 int32_t bins_per_rlist, int32_t self_interaction, int32_t half, int32_t jbits,
+                 int32_t tpa, int32_t kmax, int64_t ld, int32_t bin_cap, void* ws,
+                 uint32_t* nbr, int32_t* count, int32_t* order, int32_t* status,
+                 const int32_t* gate, void* stream) {
+  TSX_REQUIRE(n_systems > 0, "n_systems must be positive");
+  TSX_REQUIRE(n_atoms >= 0 && n_atoms < (1ll << 31), "n_atoms out of range");
+  TSX_REQUIRE(pos_stride == 3 || pos_stride == 4, "pos_stride must be 3 or 4");
+  TSX_REQUIRE(jbits >= 1 && jbits <= 26, "jbits must be in [1, 26]");
+  TSX_REQUIRE(tpa == 1 || tpa == 2 || tpa == 4 || tpa == 8 || tpa == 16,
+              "tpa must be a power of two <= 16");
+  TSX_REQUIRE(ld >= n_atoms, "ld must be >= n_atoms");
+  TSX_REQUIRE(kmax >= 0, "kmax must be >= 0");
+  TSX_REQUIRE(r_list > 0.0, "r_list must be positive");
+  TSX_REQUIRE(bins_per_rlist >= 1 && bins_per_rlist <= 4, "bins_per_rlist in [1,4]");
+  TSX_REQUIRE(bin_cap >= 1, "bin_cap must be >= 1");
+  hipStream_t st = (hipStream_t)stream;
+  NLWs w;
+  ws_layout(n_atoms, n_systems, bin_cap, &w, (char*)ws);
+  const int zero_blocks = blocks_for((int64_t)bin_cap + 1, 256);
+  k_setup<<<zero_blocks < 1024 ? zero_blocks : 1024, 1024, 0, st>>>(
+      cell_col, pbc, pbc_per_system, n_systems, r_list, bins_per_rlist, bin_cap, w.geom,
+      w.grid, w.meta, status, w.bin_count, gate);
+  TSX_CHECK_LAUNCH();
+  if (n_atoms == 0) return TSX_OK;
+  const int nb = blocks_for(n_atoms, 256);
+  k_bin<<<nb, 256, 0, st>>>(pos, pos_stride, atom_sys, n_atoms, w.geom, w.grid, w.meta,
+                            bin_cap, w.bin_count, w.atom_bin, w.slot, gate);
+  TSX_CHECK_LAUNCH();
+  k_scan_1block<int32_t, int32_t><<<1, 1024, 0, st>>>(w.bin_count, (int64_t)bin_cap, w.bin_start, gate);
+  TSX_CHECK_LAUNCH();
+  k_scatter<<<nb, 256, 0, st>>>(n_atoms, atom_sys, sys_start, w.grid, w.meta, bin_cap,
+                                w.atom_bin, w.slot, w.bin_start, order, gate);
+  TSX_CHECK_LAUNCH();
+  k_sort_bins<<<blocks_for(bin_cap, 256), 256, 0, st>>>(w.meta, bin_cap, w.bin_start, order, gate);
+  TSX_CHECK_LAUNCH();
+  k_wrap_sorted<<<nb, 256, 0, st>>>(pos, pos_stride, atom_sys, n_atoms, w.geom, w.grid, w.meta,
+                                    bin_cap, order, w.wr, gate);
+  TSX_CHECK_LAUNCH();
+  k_fill<<<nb, 256, 0, st>>>(atom_sys, sys_start, n_atoms, w.geom, w.grid, w.meta, bin_cap,
+                             w.bin_start, w.atom_bin, order, w.wr, r_list * r_list, self_interaction, half,
+                             jbits, tpa, kmax, ld, nbr, count, status, gate);
+  TSX_CHECK_LAUNCH();
+  return TSX_OK;
+}
+
+int64_t tsx_scan_ws_bytes(int64_t n) {
+  const int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  return (int64_t)sizeof(int64_t) * (2 * tiles + 4);
+}
+
+int tsx_nl_count_pairs(const int32_t* count, int64_t n, int64_t* row_ptr, void* ws,
+                       void* stream) {
+  TSX_REQUIRE(n >= 0, "n must be >= 0");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (tiles <= 1) {
+    k_scan_1block<int32_t, int64_t><<<1, 1024, 0, st>>>(count, n, row_ptr, nullptr);
+    TSX_CHECK_LAUNCH();
+    return TSX_OK;
+  }
+  TSX_REQUIRE(ws != nullptr, "workspace required for n > %d", SCAN_TILE);
+  int64_t* part = (int64_t*)ws;
+  int64_t* part_scan = part + tiles + 1;
+  k_scan_partials<<<(int)tiles, SCAN_THREADS, 0, st>>>(count, n, part);
+  TSX_CHECK_LAUNCH();
+  k_scan_1block<int64_t, int64_t><<<1, 1024, 0, st>>>(part, tiles, part_scan, nullptr);
+  TSX_CHECK_LAUNCH();
+  k_scan_final<<<(int)tiles, SCAN_THREADS, 0, st>>>(count, n, part_scan, row_ptr);
+  TSX_CHECK_LAUNCH();
+  return TSX_OK;
+}
+
+int tsx_nl_emit_pairs(const uint32_t* nbr, const int32_t* count, const int32_t* order,
+                      const int64_t* row_ptr, const int32_t* sys_start, const int32_t* atom_sys,
+                      int64_t n_atoms, int32_t jbits, int32_t tpa, int64_t ld, int64_t* mapping,
+                      int64_t n_pairs, int64_t* system_mapping, double* shifts, void* stream) {
+  if (n_atoms == 0 || n_pairs == 0) return TSX_OK;
+  hipStream_t st = (hipStream_t)stream;
+  k_emit<<<blocks_for(n_atoms, 256), 256, 0, st>>>(nbr, count, order, row_ptr, sys_start,
+                                                   atom_sys, n_atoms, jbits, tpa, ld, mapping,
+                                                   n_pairs, system_mapping, shifts);
+  TSX_CHECK_LAUNCH();
+  return TSX_OK;
+}
+
+int64_t tsx_wrap_ws_bytes(int32_t n_systems) {
+  return (int64_t)ws_layout(0, n_systems, 1, nullptr, nullptr);
+}
+
+int tsx_wrap_positions(const double* pos, const double* cell_col, const uint8_t* pbc,
+                       int32_t pbc_per_system, const int32_t* atom_sys, int64_t n_atoms,
+                       int32_t n_systems, void* ws, int32_t* status, double* wrapped,
+                       double* shifts, void* stream) {
+  TSX_REQUIRE(n_systems > 0, "n_systems must be positive");
+  hipStream_t st = (hipStream_t)stream;
+  NLWs w;
+  ws_layout(0, n_systems, 1, &w, (char*)ws);
+  k_setup<<<1, 1024, 0, st>>>(cell_col, pbc, pbc_per_system, n_systems, 1.0, 1, 1 << 30,
+                              w.geom, w.grid, w.meta, status, w.bin_count, nullptr);
+  TSX_CHECK_LAUNCH();
+  if (n_atoms == 0) return TSX_OK;
+  k_wrap_out<<<blocks_for(n_atoms, 256), 256, 0, st>>>(pos, w.geom, w.grid, atom_sys, n_atoms,
+                                                       wrapped, shifts);
+  TSX_CHECK_LAUNCH();
+  return TSX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,198 @@
+// FP32 pair functions on FP64 displacement vectors, shared by the model,
+// fused-step and backward kernels.
+#pragma once
+
+#include "tsx_common.h"
+
+namespace tsx {
+
+// Derived per-species-pair constants in FP32:
+//   LJ:          a = sigma^2, b = 4 eps,  c = 24 eps
+//   Morse:       a = sigma,   b = eps,    c = alpha,  d = 2 eps alpha
+//   soft sphere: a = sigma,   b = eps,    c = alpha,  d = eps / sigma
+struct PairP {
+  float a, b, c, d;
+};
+
+__host__ __device__ inline PairP make_pair_params(int kind, double sigma, double eps,
+                                                  double alpha) {
+  PairP p;
+  if (kind == TSX_LJ) {
+    p.a = (float)(sigma * sigma);
+    p.b = (float)(4.0 * eps);
+    p.c = (float)(24.0 * eps);
+    p.d = 0.f;
+  } else if (kind == TSX_MORSE) {
+    p.a = (float)sigma;
+    p.b = (float)eps;
+    p.c = (float)alpha;
+    p.d = (float)(2.0 * eps * alpha);
+  } else {
+    p.a = (float)sigma;
+    p.b = (float)eps;
+    p.c = (float)alpha;
+    p.d = (float)(eps / sigma);
+  }
+  return p;
+}
+
+// Energy e and scalar fs with force vector f_ij = fs * dr_ij = -(dV/dr / r) dr_ij
+// (the force on j due to i, pair_potential.py:426-428).
+template <int KIND>
+__device__ __forceinline__ void pair_eval(float r2, const PairP& p, float& e, float& fs) {
+  if (KIND == TSX_LJ) {
+    if (r2 > 0.f) {
+      const float ir2 = 1.0f / r2;
+      const float sr2 = p.a * ir2;
+      const float sr6 = sr2 * sr2 * sr2;
+      e = p.b * (sr6 * sr6 - sr6);
+      fs = p.c * (2.f * sr6 * sr6 - sr6) * ir2;
+    } else {
+      e = 0.f;
+      fs = 0.f;
+    }
+  } else if (KIND == TSX_MORSE) {
+    if (r2 > 0.f) {
+      const float r = __fsqrt_rn(r2);
+      const float x = __expf(-p.c * (r - p.a));
+      const float omx = 1.f - x;
+      e = p.b * omx * omx - p.b;
+      fs = -p.d * x * omx / r;
+    } else {
+      e = 0.f;
+      fs = 0.f;
+    }
+  } else {
+    const float r = __fsqrt_rn(r2);
+    if (r < p.a) {
+      const float t = 1.f - r / p.a;
+      if (p.c == 2.f) {
+        e = 0.5f * p.b * t * t;
+        fs = r > 0.f ? p.d * t / r : 0.f;
+      } else {
+        const float tm1 = powf(t, p.c - 1.f);
+        e = p.b / p.c * tm1 * t;
+        fs = r > 0.f ? p.d * tm1 / r : 0.f;
+      }
+    } else {
+      e = 0.f;
+      fs = 0.f;
+    }
+  }
+}
+
+// Second radial derivative helpers for the backward kernels:
+//   phi'(r) and phi''(r) in FP32 together with dphi'/dsigma, dphi'/deps.
+template <int KIND>
+__device__ __forceinline__ void pair_derivs(float r, const PairP& p, float& d1, float& d2) {
+  if (KIND == TSX_LJ) {
+    // V = 4e (s^12 r^-12 - s^6 r^-6)
+    const float ir = 1.f / r;
+    const float sr2 = p.a * ir * ir;
+    const float sr6 = sr2 * sr2 * sr2;
+    const float e4 = p.b;
+    d1 = e4 * (-12.f * sr6 * sr6 + 6.f * sr6) * ir;
+    d2 = e4 * (156.f * sr6 * sr6 - 42.f * sr6) * ir * ir;
+  } else if (KIND == TSX_MORSE) {
+    const float x = __expf(-p.c * (r - p.a));
+    d1 = 2.f * p.b * p.c * x * (1.f - x);
+    d2 = 2.f * p.b * p.c * p.c * x * (2.f * x - 1.f);
+  } else {
+    if (r < p.a) {
+      const float t = 1.f - r / p.a;
+      const float tm2 = (p.c == 2.f) ? 1.f : powf(t, p.c - 2.f);
+      d1 = -p.d * tm2 * t;
+      d2 = p.d / p.a * (p.c - 1.f) * tm2;
+    } else {
+      d1 = 0.f;
+      d2 = 0.f;
+    }
+  }
+}
+
+}  // namespace tsx
+
+namespace tsx {
+
+__device__ __forceinline__ int64_t ell_idx(int64_t t, int n, int tpa, int64_t ld) {
+  return (int64_t)(n / tpa) * (ld * tpa) + t * tpa + (n % tpa);
+}
+
+struct RowAcc {
+  double fx, fy, fz, e;
+  double w[6];
+};
+
+// Walk this lane's share of row t (atom i) of the ELL list and accumulate the
+// FP64 force on i, half the pair energies and half the pair virials.  The
+// displacement is (r_j - r_i) + S.L in FP64 (pair_potential.py:181-183), the
+// mask is d^2 < rc^2 (pair_potential.py:194), the pair function FP32.
+// Species of atom j come from ``species`` (stride 1) or, when ``species`` is
+// NULL and ns > 1, from the packed fourth coordinate of a stride-4 position.
+template <int KIND, int TPA, bool VIRIAL, int PS>
+__device__ __forceinline__ RowAcc accumulate_row(
+    const PairP* sp, int ns, const int32_t* __restrict__ species,
+    const double* __restrict__ pos, const double* C, int base, double xi, double yi, double zi,
+    int ti, const uint32_t* __restrict__ nbr, int64_t t, int cnt, int lane, int jbits,
+    int64_t ld, double rc2) {
+  RowAcc a;
+  a.fx = a.fy = a.fz = a.e = 0.0;
+  for (int c = 0; c < 6; ++c) a.w[c] = 0.0;
+  const double L00 = C[0], L01 = C[3], L02 = C[6];
+  const double L10 = C[1], L11 = C[4], L12 = C[7];
+  const double L20 = C[2], L21 = C[5], L22 = C[8];
+  const int sbits = (32 - jbits) / 3;
+  const int bias = 1 << (sbits - 1);
+  const uint32_t fmask = (1u << sbits) - 1u;
+  const uint32_t jmask = (1u << jbits) - 1u;
+  for (int n = lane; n < cnt; n += TPA) {
+    const uint32_t w = nbr[ell_idx(t, n, TPA, ld)];
+    const int j = base + (int)(w & jmask);
+    const uint32_t sf = w >> jbits;
+    const int sx = (int)(sf & fmask) - bias;
+    const int sy = (int)((sf >> sbits) & fmask) - bias;
+    const int sz = (int)((sf >> (2 * sbits)) & fmask) - bias;
+    const double* pj = pos + PS * (int64_t)j;
+    double dx = pj[0] - xi;
+    double dy = pj[1] - yi;
+    double dz = pj[2] - zi;
+    if ((sx | sy | sz) != 0) {
+      const double fxs = (double)sx, fys = (double)sy, fzs = (double)sz;
+      dx += fxs * L00 + fys * L10 + fzs * L20;
+      dy += fxs * L01 + fys * L11 + fzs * L21;
+      dz += fxs * L02 + fys * L12 + fzs * L22;
+    }
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (!(d2 < rc2)) continue;
+    int tj = 0;
+    if (ns > 1) tj = species ? species[j] : (int)__double_as_longlong(pj[3]);
+    float e, fs;
+    pair_eval<KIND>((float)d2, sp[ti * ns + tj], e, fs);
+    const double f = (double)fs;
+    const double fvx = f * dx, fvy = f * dy, fvz = f * dz;
+    a.fx -= fvx;
+    a.fy -= fvy;
+    a.fz -= fvz;
+    a.e += 0.5 * (double)e;
+    if (VIRIAL) {
+      a.w[0] += 0.5 * dx * fvx;
+      a.w[1] += 0.5 * dy * fvy;
+      a.w[2] += 0.5 * dz * fvz;
+      a.w[3] += 0.5 * dy * fvz;
+      a.w[4] += 0.5 * dx * fvz;
+      a.w[5] += 0.5 * dx * fvy;
+    }
+  }
+#pragma unroll
+  for (int off = TPA / 2; off > 0; off >>= 1) {
+    a.fx += __shfl_xor(a.fx, off, 64);
+    a.fy += __shfl_xor(a.fy, off, 64);
+    a.fz += __shfl_xor(a.fz, off, 64);
+    a.e += __shfl_xor(a.e, off, 64);
+    if (VIRIAL)
+      for (int c = 0; c < 6; ++c) a.w[c] += __shfl_xor(a.w[c], off, 64);
+  }
+  return a;
+}
+
+}  // namespace tsx

@@ -1,0 +1,328 @@
+"""Neighbor lists on the GPU through the tsx C ABI.
+
+``torchsim_nl`` keeps the reference signature and output contract
+(torch_sim/neighbors/__init__.py:46-98, torch_nl.py:156-215):
+
+    torchsim_nl(positions [N,3], cell [S,3,3] row-vector, pbc [S,3] or [3],
+                cutoff, system_idx [N], self_interaction=False)
+      -> (mapping [2,P] int64, system_mapping [P] int64, shifts_idx [P,3])
+
+with the full ordered pair set {(i, j, S): |r_j - r_i + S.L| < cutoff},
+(i, i, 0) excluded unless ``self_interaction``, shifts valid for the
+unwrapped input positions.  The order of the pairs is the build's row order
+(the reference order is unspecified, transforms.py:999, 1291).
+
+:class:`NeighborList` owns the device buffers of one list (ELL rows, counts,
+the bin-sorted atom order, workspace and the device status word) and is
+reused by the models and the batched engine.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call, ptr, stream
+
+STATUS_NL_OVERFLOW = 1
+STATUS_BIN_OVERFLOW = 2
+STATUS_SHIFT_OVERFLOW = 4
+STATUS_SINGULAR_CELL = 8
+
+
+class SystemIndex:
+    """Device bookkeeping of the system layout: int32 atom->system map and the
+    [S+1] atom offsets of each system (no host synchronisation)."""
+
+    def __init__(self, system_idx: torch.Tensor, n_systems: int):
+        dev = system_idx.device
+        self.n_systems = int(n_systems)
+        self.n_atoms = int(system_idx.shape[0])
+        self.atom_sys = system_idx.to(torch.int32).contiguous()
+        bounds = torch.arange(self.n_systems + 1, device=dev, dtype=system_idx.dtype)
+        self.sys_start = torch.searchsorted(system_idx, bounds).to(torch.int32).contiguous()
+        self._key = (system_idx.data_ptr(), system_idx._version, self.n_atoms, self.n_systems)
+        self._counts = None
+
+    def matches(self, system_idx: torch.Tensor, n_systems: int) -> bool:
+        return self._key == (
+            system_idx.data_ptr(),
+            system_idx._version,
+            int(system_idx.shape[0]),
+            int(n_systems),
+        )
+
+    @property
+    def counts(self) -> np.ndarray:
+        """Atoms per system on the host (one synchronisation, cached)."""
+        if self._counts is None:
+            self._counts = (self.sys_start[1:] - self.sys_start[:-1]).cpu().numpy()
+        return self._counts
+
+
+def jbits_for(max_atoms_per_system: int) -> int:
+    """Bits for a system-local atom index."""
+    return max(1, int(math.ceil(math.log2(max(2, max_atoms_per_system)))))
+
+
+def plan_capacity(
+    cell_col: np.ndarray,
+    pbc: np.ndarray,
+    counts: np.ndarray,
+    r_list: float,
+    bins_per_rlist: int,
+) -> tuple[int, int]:
+    """Host estimate of (bin count, per-atom neighbor capacity).
+
+    Mirrors the device grid rule nb_k = max(1, floor(q w_k / r_list)) on
+    periodic axes; the neighbor estimate is 1.3 rho (4/3) pi r^3 + 24 for
+    periodic systems and n_s for systems without periodicity.
+    """
+    n_sys = cell_col.shape[0]
+    pbc = np.broadcast_to(pbc.reshape(-1, 3), (n_sys, 3))
+    total_bins = 0
+    kmax = 1
+    for s in range(n_sys):
+        lrow = cell_col[s].T
+        det = np.linalg.det(lrow)
+        active = bool(pbc[s].any()) and np.isfinite(det) and abs(det) > 1e-12
+        nb = [1, 1, 1]
+        if active:
+            inv = np.linalg.inv(lrow)
+            w = 1.0 / np.linalg.norm(inv, axis=0)
+            for k in range(3):
+                if pbc[s, k]:
+                    nb[k] = int(min(1024, max(1, math.floor(bins_per_rlist * w[k] / r_list))))
+            rho = counts[s] / abs(det)
+            k_est = 1.3 * rho * 4.0 / 3.0 * math.pi * r_list**3 + 24
+            if not pbc[s].all():
+                k_est = max(k_est, float(counts[s]))
+        else:
+            k_est = float(counts[s])
+        total_bins += nb[0] * nb[1] * nb[2]
+        kmax = max(kmax, int(math.ceil(k_est)))
+    return total_bins, kmax
+
+
+class NeighborList:
+    """Device neighbor list (ELL rows in bin order) with overflow handling."""
+
+    def __init__(
+        self,
+        r_list: float,
+        *,
+        half: bool = False,
+        self_interaction: bool = False,
+        tpa: int = 4,
+        bins_per_rlist: int = 1,
+    ):
+        self.r_list = float(r_list)
+        self.half = bool(half)
+        self.self_interaction = bool(self_interaction)
+        self.tpa = int(tpa)
+        self.bins_per_rlist = int(bins_per_rlist)
+        self.kmax = 0
+        self.bin_cap = 0
+        self.jbits = 1
+        self.n_atoms = -1
+        self.n_systems = -1
+        self.nbr = self.count = self.order = self.ws = self.status = None
+
+    # ------------------------------------------------------------ sizing
+    def ensure(self, n_atoms: int, n_systems: int, kmax: int, bin_cap: int, jbits: int,
+               device: torch.device) -> None:
+        """(Re)allocate buffers for at least the given capacities."""
+        kmax = max(1, kmax)
+        kmax = ((kmax + self.tpa - 1) // self.tpa) * self.tpa
+        realloc = (
+            self.nbr is None
+            or n_atoms != self.n_atoms
+            or n_systems != self.n_systems
+            or kmax > self.kmax
+            or bin_cap > self.bin_cap
+        )
+        self.jbits = jbits
+        if not realloc:
+            return
+        self.kmax = max(kmax, self.kmax if n_atoms == self.n_atoms else 0)
+        self.bin_cap = max(bin_cap, self.bin_cap if n_systems == self.n_systems else 0)
+        self.n_atoms, self.n_systems = n_atoms, n_systems
+        rows = max(1, n_atoms)
+        self.nbr = torch.empty(
+            (self.kmax // self.tpa) * rows * self.tpa, dtype=torch.int32, device=device
+        )
+        self.count = torch.zeros(rows, dtype=torch.int32, device=device)
+        self.order = torch.zeros(rows, dtype=torch.int32, device=device)
+        wsb = int(_lib.lib().tsx_nl_ws_bytes(n_atoms, n_systems, self.bin_cap))
+        self.ws = torch.empty(wsb, dtype=torch.uint8, device=device)
+        self.status = torch.zeros(1, dtype=torch.int32, device=device)
+
+    @property
+    def ld(self) -> int:
+        return max(1, self.n_atoms)
+
+    # ------------------------------------------------------------ building
+    def enqueue(
+        self,
+        pos: torch.Tensor,
+        pos_stride: int,
+        cell_col: torch.Tensor,
+        pbc_u8: torch.Tensor,
+        pbc_per_system: bool,
+        sysidx: SystemIndex,
+        gate: torch.Tensor | None = None,
+    ) -> None:
+        """Enqueue a build on the current stream (capturable, no host sync)."""
+        call(
+            "tsx_nl_build",
+            ptr(pos), pos_stride, ptr(cell_col), ptr(pbc_u8), int(pbc_per_system),
+            ptr(sysidx.sys_start), ptr(sysidx.atom_sys), sysidx.n_atoms, sysidx.n_systems,
+            self.r_list, self.bins_per_rlist, int(self.self_interaction), int(self.half),
+            self.jbits, self.tpa, self.kmax, self.ld, self.bin_cap, ptr(self.ws),
+            ptr(self.nbr), ptr(self.count), ptr(self.order), ptr(self.status), ptr(gate),
+            stream(),
+        )
+
+    def build(
+        self,
+        pos: torch.Tensor,
+        cell_col: torch.Tensor,
+        pbc: torch.Tensor,
+        sysidx: SystemIndex,
+        pos_stride: int = 3,
+        max_retries: int = 6,
+    ) -> None:
+        """Build with host-side capacity planning and overflow retries."""
+        _lib.require_gpu(pos, "positions")
+        pbc_per_system = pbc.ndim == 2
+        pbc_u8 = pbc.to(torch.uint8).contiguous()
+        counts = sysidx.counts
+        jbits = jbits_for(int(counts.max()) if counts.size else 1)
+        if self.nbr is None or sysidx.n_atoms != self.n_atoms or sysidx.n_systems != self.n_systems:
+            bins, kmax = plan_capacity(
+                cell_col.detach().cpu().numpy(),
+                pbc.detach().cpu().numpy(),
+                counts,
+                self.r_list,
+                self.bins_per_rlist,
+            )
+            self.ensure(sysidx.n_atoms, sysidx.n_systems, kmax, bins + 64, jbits, pos.device)
+        self.jbits = jbits
+        for _ in range(max_retries):
+            self.status.zero_()
+            self.enqueue(pos, pos_stride, cell_col, pbc_u8, pbc_per_system, sysidx)
+            st = int(self.status.item())
+            if st & STATUS_SINGULAR_CELL:
+                raise _lib.TsxError("periodic system with a singular cell")
+            if st & STATUS_SHIFT_OVERFLOW:
+                raise _lib.TsxError(
+                    "image shift outside the packed range: positions are too far "
+                    "outside their cells; wrap them first"
+                )
+            if not st & (STATUS_NL_OVERFLOW | STATUS_BIN_OVERFLOW):
+                return
+            kmax = self.kmax * 2 if st & STATUS_NL_OVERFLOW else self.kmax
+            bins = self.bin_cap * 2 if st & STATUS_BIN_OVERFLOW else self.bin_cap
+            self.ensure(sysidx.n_atoms, sysidx.n_systems, kmax, bins, jbits, pos.device)
+        raise _lib.TsxError("neighbor list capacity could not be resolved")
+
+    # ----------------------------------------------------------- exporting
+    def pairs(self, sysidx: SystemIndex, dtype: torch.dtype):
+        """Reference triple (mapping, system_mapping, shifts) from the list."""
+        dev = self.count.device
+        n = sysidx.n_atoms
+        if n == 0:
+            return (
+                torch.zeros((2, 0), dtype=torch.long, device=dev),
+                torch.zeros(0, dtype=torch.long, device=dev),
+                torch.zeros((0, 3), dtype=dtype, device=dev),
+            )
+        row_ptr = torch.empty(n + 1, dtype=torch.int64, device=dev)
+        ws = torch.empty(int(_lib.lib().tsx_scan_ws_bytes(n)), dtype=torch.uint8, device=dev)
+        call("tsx_nl_count_pairs", ptr(self.count), n, ptr(row_ptr), ptr(ws), stream())
+        n_pairs = int(row_ptr[n].item())
+        mapping = torch.empty((2, n_pairs), dtype=torch.int64, device=dev)
+        sysmap = torch.empty(n_pairs, dtype=torch.int64, device=dev)
+        shifts = torch.empty((n_pairs, 3), dtype=torch.float64, device=dev)
+        if n_pairs:
+            call(
+                "tsx_nl_emit_pairs",
+                ptr(self.nbr), ptr(self.count), ptr(self.order), ptr(row_ptr),
+                ptr(sysidx.sys_start), ptr(sysidx.atom_sys), n, self.jbits, self.tpa, self.ld,
+                ptr(mapping), n_pairs, ptr(sysmap), ptr(shifts), stream(),
+            )
+        return mapping, sysmap, shifts.to(dtype)
+
+
+def _normalize(cell: torch.Tensor, pbc: torch.Tensor, n_systems: int):
+    """Cell -> [S,3,3], pbc -> [S,3] (torch_sim/neighbors/utils.py:7-34)."""
+    if cell.ndim == 2:
+        cell = cell.unsqueeze(0).expand(n_systems, -1, -1) if cell.shape[0] == 3 else cell.reshape(n_systems, 3, 3)
+    if pbc.ndim == 1:
+        pbc = pbc.unsqueeze(0).expand(n_systems, -1) if pbc.shape[0] == 3 else pbc.reshape(n_systems, 3)
+    return cell, pbc
+
+
+def torchsim_nl(
+    positions: torch.Tensor,
+    cell: torch.Tensor,
+    pbc: torch.Tensor,
+    cutoff: float,
+    system_idx: torch.Tensor,
+    self_interaction: bool = False,  # noqa: FBT001, FBT002
+) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """Full neighbor list with the reference contract (neighbors/__init__.py:46-98).
+
+    ``cell`` uses the row-vector convention of the reference NL functions.
+    """
+    _lib.require_gpu(positions, "positions")
+    dtype = positions.dtype
+    if positions.shape[0] == 0:
+        dev = positions.device
+        return (
+            torch.zeros((2, 0), dtype=torch.long, device=dev),
+            torch.zeros(0, dtype=torch.long, device=dev),
+            torch.zeros((0, 3), dtype=dtype, device=dev),
+        )
+    n_systems = int(system_idx.max().item()) + 1
+    cell, pbc = _normalize(cell, pbc, n_systems)
+    cell_col = cell.to(torch.float64).transpose(1, 2).contiguous()
+    pos = positions.to(torch.float64).contiguous()
+    sysidx = SystemIndex(system_idx, n_systems)
+    nl = NeighborList(cutoff, self_interaction=self_interaction, tpa=1)
+    nl.build(pos, cell_col, pbc.contiguous(), sysidx)
+    return nl.pairs(sysidx, dtype)
+
+
+# The reference exposes the same contract under several names; on this
+# backend all of them are the HIP cell-list search.
+torch_nl_linked_cell = torchsim_nl
+torch_nl_n2 = torchsim_nl
+default_batched_nl = torchsim_nl
+
+
+def wrap_positions(
+    positions: torch.Tensor, cell_row: torch.Tensor, system_idx: torch.Tensor, pbc: torch.Tensor
+) -> tuple[torch.Tensor, torch.Tensor]:
+    """Wrapped positions and integer shifts (transforms.py:160-210) on the GPU."""
+    _lib.require_gpu(positions, "positions")
+    n_systems = cell_row.shape[0]
+    if pbc.ndim == 1:
+        pbc = pbc.unsqueeze(0).expand(n_systems, -1)
+    dev = positions.device
+    cell_col = cell_row.to(torch.float64).transpose(1, 2).contiguous()
+    pos = positions.to(torch.float64).contiguous()
+    atom_sys = system_idx.to(torch.int32).contiguous()
+    ws = torch.empty(int(_lib.lib().tsx_wrap_ws_bytes(n_systems)), dtype=torch.uint8, device=dev)
+    status = torch.zeros(1, dtype=torch.int32, device=dev)
+    wrapped = torch.empty_like(pos)
+    shifts = torch.empty_like(pos)
+    call(
+        "tsx_wrap_positions", ptr(pos), ptr(cell_col), ptr(pbc.to(torch.uint8).contiguous()), 1,
+        ptr(atom_sys), pos.shape[0], n_systems, ptr(ws), ptr(status), ptr(wrapped), ptr(shifts),
+        stream(),
+    )
+    return wrapped.to(positions.dtype), shifts.to(positions.dtype)

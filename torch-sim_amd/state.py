@@ -1,0 +1,231 @@
+"""Batched simulation state, mirroring torch_sim.state.SimState.
+
+Field layout and conventions follow torch_sim/state.py:120-326:
+``positions [N,3]``, ``masses [N]``, ``cell [S,3,3]`` in the *column-vector*
+convention (lattice vector k is ``cell[s, :, k]``), ``pbc [3]`` bool shared by
+all systems, ``atomic_numbers [N]`` and ``system_idx [N]`` (int64, sorted,
+consecutive from 0).  ``n_systems`` is ``cell.shape[0]`` (the reference
+computes it with ``torch.unique``, state.py:437-440; the invariant
+``cell.shape[0] == n_systems`` is validated at construction, so both agree
+without a device synchronisation).
+"""
+
+from __future__ import annotations
+
+import copy
+from dataclasses import dataclass, field, fields
+from typing import Any, ClassVar
+
+import torch
+
+
+@dataclass(kw_only=True)
+class SimState:
+    """State of a batch of periodic atomistic systems (state.py:120-326)."""
+
+    positions: torch.Tensor
+    masses: torch.Tensor
+    cell: torch.Tensor
+    pbc: torch.Tensor | list | bool
+    atomic_numbers: torch.Tensor
+    system_idx: torch.Tensor | None = None
+    _rng: Any = field(default=None, repr=False)
+
+    _atom_attributes: ClassVar[set[str]] = {
+        "positions",
+        "masses",
+        "atomic_numbers",
+        "system_idx",
+    }
+    _system_attributes: ClassVar[set[str]] = {"cell"}
+    _global_attributes: ClassVar[set[str]] = {"pbc", "_rng"}
+
+    def __post_init__(self) -> None:
+        if not isinstance(self.pbc, torch.Tensor):
+            v = [self.pbc] * 3 if isinstance(self.pbc, bool) else list(self.pbc)
+            self.pbc = torch.tensor(v, dtype=torch.bool, device=self.positions.device)
+        if self.system_idx is None:
+            self.system_idx = torch.zeros(
+                self.positions.shape[0], dtype=torch.int64, device=self.positions.device
+            )
+        if self.cell.ndim == 2:
+            self.cell = self.cell.unsqueeze(0)
+        n = {self.positions.shape[0], self.masses.shape[0], self.atomic_numbers.shape[0]}
+        if len(n) > 1:
+            raise ValueError(
+                f"Incompatible shapes: positions {self.positions.shape[0]}, "
+                f"masses {self.masses.shape[0]}, atomic_numbers {self.atomic_numbers.shape[0]}"
+            )
+        if self.cell.shape[-2:] != (3, 3):
+            raise ValueError("Cell must have shape (n_systems, 3, 3)")
+
+    # ------------------------------------------------------------- basics
+    @property
+    def device(self) -> torch.device:
+        return self.positions.device
+
+    @property
+    def dtype(self) -> torch.dtype:
+        return self.positions.dtype
+
+    @property
+    def n_atoms(self) -> int:
+        return int(self.positions.shape[0])
+
+    @property
+    def n_systems(self) -> int:
+        return int(self.cell.shape[0])
+
+    @property
+    def n_atoms_per_system(self) -> torch.Tensor:
+        return torch.bincount(self.system_idx, minlength=self.n_systems)
+
+    @property
+    def row_vector_cell(self) -> torch.Tensor:
+        return self.cell.mT
+
+    @row_vector_cell.setter
+    def row_vector_cell(self, value: torch.Tensor) -> None:
+        self.cell = value.mT
+
+    @property
+    def column_vector_cell(self) -> torch.Tensor:
+        return self.cell
+
+    @property
+    def volume(self) -> torch.Tensor:
+        return torch.det(self.cell)
+
+    @property
+    def rng(self) -> torch.Generator:
+        """Per-state generator on the state's device (state.py:186, 213-221)."""
+        if not isinstance(self._rng, torch.Generator):
+            g = torch.Generator(device=self.device)
+            if self._rng is None:
+                g.seed()
+            else:
+                g.manual_seed(int(self._rng))
+            self._rng = g
+        return self._rng
+
+    @rng.setter
+    def rng(self, value: Any) -> None:
+        self._rng = value
+
+    def get_number_of_degrees_of_freedom(self) -> torch.Tensor:
+        """3 n per system (no constraints; constraints.py:691-699)."""
+        return 3 * self.n_atoms_per_system
+
+    # ------------------------------------------------------ manipulation
+    def _field_names(self) -> list[str]:
+        return [f.name for f in fields(self)]
+
+    def clone(self) -> SimState:
+        kw = {}
+        for name in self._field_names():
+            v = getattr(self, name)
+            if isinstance(v, torch.Tensor):
+                kw[name] = v.clone()
+            elif name == "_rng":
+                kw[name] = v if not isinstance(v, torch.Generator) else None
+            else:
+                kw[name] = copy.copy(v)
+        out = type(self)(**kw)
+        if isinstance(self._rng, torch.Generator):
+            g = torch.Generator(device=self.device)
+            g.set_state(self._rng.get_state())
+            out._rng = g
+        return out
+
+    def to(self, device: torch.device | str | None = None, dtype: torch.dtype | None = None):
+        kw = {}
+        for name in self._field_names():
+            v = getattr(self, name)
+            if isinstance(v, torch.Tensor):
+                if v.is_floating_point() and dtype is not None:
+                    v = v.to(device=device, dtype=dtype)
+                else:
+                    v = v.to(device=device)
+            elif name == "_rng":
+                v = None if isinstance(v, torch.Generator) else v
+            kw[name] = v
+        return type(self)(**kw)
+
+    def set_cell(self, cell: torch.Tensor, scale_atoms: bool = False) -> None:
+        """Set the cell, optionally keeping fractional coordinates (state.py:537-560)."""
+        if cell.shape != self.cell.shape:
+            raise ValueError(f"New cell must have shape {self.cell.shape}, got {cell.shape}")
+        if scale_atoms:
+            m = torch.linalg.solve(self.cell.mT, cell.mT)
+            self.positions = torch.bmm(self.positions.unsqueeze(1), m[self.system_idx]).squeeze(1)
+        self.cell = cell
+
+    def store_model_extras(self, model_output: dict[str, torch.Tensor]) -> None:
+        """Accepted for interface compatibility (state.py:376); extras are ignored."""
+
+    @classmethod
+    def from_state(cls, state: SimState, **kwargs: Any):
+        """New state of class ``cls`` from the fields of ``state`` plus ``kwargs``."""
+        kw = {n: getattr(state, n) for n in state._field_names() if n in {f.name for f in fields(cls)}}
+        kw.update(kwargs)
+        return cls(**kw)
+
+    def split(self) -> list[SimState]:
+        """One state per system (state.py:1140-1315)."""
+        counts = self.n_atoms_per_system.tolist()
+        out, a = [], 0
+        for s, c in enumerate(counts):
+            kw = {}
+            for name in self._field_names():
+                v = getattr(self, name)
+                if name in self._atom_attributes and isinstance(v, torch.Tensor):
+                    v = torch.zeros_like(v[a : a + c]) if name == "system_idx" else v[a : a + c]
+                elif name in self._system_attributes and isinstance(v, torch.Tensor):
+                    v = v[s : s + 1]
+                elif name == "_rng":
+                    v = None
+                kw[name] = v
+            out.append(type(self)(**kw))
+            a += c
+        return out
+
+
+def concatenate_states(states: list[SimState]) -> SimState:
+    """Concatenate states along atoms and systems (state.py:1317-1477)."""
+    if not states:
+        raise ValueError("Cannot concatenate an empty list of states")
+    first = states[0]
+    cls = type(first)
+    kw: dict[str, Any] = {}
+    offset = 0
+    sys_idx = []
+    for st in states:
+        sys_idx.append(st.system_idx + offset)
+        offset += st.n_systems
+    for f in fields(cls):
+        name = f.name
+        v0 = getattr(first, name)
+        if name == "system_idx":
+            kw[name] = torch.cat(sys_idx)
+        elif name == "pbc":
+            kw[name] = v0
+        elif name == "_rng":
+            kw[name] = None
+        elif isinstance(v0, torch.Tensor) and v0.ndim >= 1:
+            kw[name] = torch.cat([getattr(s, name) for s in states])
+        else:
+            kw[name] = v0
+    return cls(**kw)
+
+
+def initialize_state(system: SimState | list[SimState], device=None, dtype=None) -> SimState:
+    """Coerce a SimState or a list of SimStates to one state (state.py:1480-1547)."""
+    if isinstance(system, SimState):
+        return system.to(device=device, dtype=dtype) if (device or dtype) else system
+    if isinstance(system, (list, tuple)) and all(isinstance(s, SimState) for s in system):
+        st = concatenate_states(list(system))
+        return st.to(device=device, dtype=dtype) if (device or dtype) else st
+    raise TypeError(
+        "torch_sim_amd builds states from SimState objects; ASE/pymatgen conversion "
+        "is outside this package"
+    )
