@@ -105,14 +105,16 @@ __device__ __forceinline__ bool pbc_of(const uint8_t* pbc, int per_sys, int s, i
 // ------------------------------------------------------------------ setup
 __global__ void k_setup(const double* __restrict__ cell_col, const uint8_t* __restrict__ pbc,
                         int pbc_per_sys, int n_sys, double r_list, int q, int bin_cap,
+                        int zero_bins,
                         double* __restrict__ geom, int32_t* __restrict__ grid,
                         int32_t* __restrict__ meta, int32_t* __restrict__ status,
                         int32_t* __restrict__ bin_count, const int32_t* __restrict__ gate) {
   if (gate && *gate == 0) return;
-  // every block zeroes part of the bin counters
-  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b <= bin_cap;
-       b += (int64_t)gridDim.x * blockDim.x)
-    bin_count[b] = 0;
+  // every block zeroes part of the bin counters [0, bin_cap]
+  if (zero_bins)
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b <= bin_cap;
+         b += (int64_t)gridDim.x * blockDim.x)
+      bin_count[b] = 0;
   if (blockIdx.x != 0) return;
   __shared__ int32_t sh_nb[1024];
   __shared__ int32_t sh_carry;
@@ -188,7 +190,7 @@ __global__ void k_setup(const double* __restrict__ cell_col, const uint8_t* __re
   }
   if (threadIdx.x == 0) {
     meta[0] = sh_carry;
-    if (sh_carry > bin_cap) atomicOr(status, TSX_STATUS_BIN_OVERFLOW);
+    if (zero_bins && sh_carry > bin_cap) atomicOr(status, TSX_STATUS_BIN_OVERFLOW);
   }
 }
 
@@ -587,7 +589,7 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
   ws_layout(n_atoms, n_systems, bin_cap, &w, (char*)ws);
   const int zero_blocks = blocks_for((int64_t)bin_cap + 1, 256);
   k_setup<<<zero_blocks < 1024 ? zero_blocks : 1024, 1024, 0, st>>>(
-      cell_col, pbc, pbc_per_system, n_systems, r_list, bins_per_rlist, bin_cap, w.geom,
+      cell_col, pbc, pbc_per_system, n_systems, r_list, bins_per_rlist, bin_cap, 1, w.geom,
       w.grid, w.meta, status, w.bin_count, gate);
   TSX_CHECK_LAUNCH();
   if (n_atoms == 0) return TSX_OK;
@@ -664,8 +666,9 @@ int tsx_wrap_positions(const double* pos, const double* cell_col, const uint8_t*
   hipStream_t st = (hipStream_t)stream;
   NLWs w;
   ws_layout(0, n_systems, 1, &w, (char*)ws);
-  k_setup<<<1, 1024, 0, st>>>(cell_col, pbc, pbc_per_system, n_systems, 1.0, 1, 1 << 30,
-                              w.geom, w.grid, w.meta, status, w.bin_count, nullptr);
+  // geometry only: no bin counters are touched (the workspace holds 2)
+  k_setup<<<1, 1024, 0, st>>>(cell_col, pbc, pbc_per_system, n_systems, 1.0, 1, 1, 0, w.geom,
+                              w.grid, w.meta, status, w.bin_count, nullptr);
   TSX_CHECK_LAUNCH();
   if (n_atoms == 0) return TSX_OK;
   k_wrap_out<<<blocks_for(n_atoms, 256), 256, 0, st>>>(pos, w.geom, w.grid, atom_sys, n_atoms,

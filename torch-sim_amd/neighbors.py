@@ -57,9 +57,18 @@ class SystemIndex:
 
     @property
     def counts(self) -> np.ndarray:
-        """Atoms per system on the host (one synchronisation, cached)."""
+        """Atoms per system on the host (one synchronisation, cached).
+
+        Also validates the layout the kernels index with: system_idx sorted,
+        within [0, n_systems), so that the offsets cover every atom."""
         if self._counts is None:
-            self._counts = (self.sys_start[1:] - self.sys_start[:-1]).cpu().numpy()
+            start = self.sys_start.cpu().numpy()
+            if self.n_atoms and (start[0] != 0 or start[-1] != self.n_atoms):
+                raise ValueError(
+                    "system_idx must be sorted with values in [0, n_systems) "
+                    f"(n_systems={self.n_systems})"
+                )
+            self._counts = np.diff(start)
         return self._counts
 
 
