@@ -95,9 +95,18 @@ const char* tsx_last_error(void);
  * half != 0 keeps one copy of every unordered pair (i < j, or i == j with a
  * lexicographically positive S'), as full_to_half_list does.
  *
- * ``gate`` (device int32, may be NULL): when it points at 0 every kernel of
- * the build returns immediately, so a captured graph can carry a rebuild that
- * only runs when the displacement test of the previous step asked for it.
+ * ``setup`` != 0 recomputes the per-system geometry (inverse cell, grid,
+ * bin offsets) and clears the bin counters; a caller whose cells do not
+ * change passes 1 on the first build only.  The workspace keeps the geometry
+ * between builds and the bin counters are left cleared after every build.
+ *
+ * ``gate`` (device int32 [S], may be NULL = all systems): only systems with
+ * gate[s] != 0 are rebuilt; their rows, order entries and bins are rewritten
+ * and every other system's list is left untouched.  A captured graph can
+ * therefore carry a rebuild that runs only for the systems whose atoms left
+ * the Verlet skin.  ``counter`` (may be NULL) is incremented once per rebuilt
+ * system.  ``xref`` (stride 4, may be NULL) receives a copy of the positions
+ * of the rebuilt systems (the displacement reference of the skin).
  */
 int64_t tsx_nl_ws_bytes(int64_t n_atoms, int32_t n_systems, int32_t bin_cap);
 
@@ -106,8 +115,9 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
                  const int32_t* atom_sys, int64_t n_atoms, int32_t n_systems,
                  double r_list, int32_t bins_per_rlist, int32_t self_interaction,
                  int32_t half, int32_t jbits, int32_t tpa, int32_t kmax, int64_t ld,
-                 int32_t bin_cap, void* ws, uint32_t* nbr, int32_t* count,
-                 int32_t* order, int32_t* status, const int32_t* gate, void* stream);
+                 int32_t bin_cap, int32_t setup, void* ws, uint32_t* nbr, int32_t* count,
+                 int32_t* order, double* xref, int32_t* status, const int32_t* gate,
+                 int32_t* counter, void* stream);
 
 /* Exclusive scan of ``count`` into ``row_ptr`` [n+1] (int64); the total is
  * row_ptr[n].  Used before tsx_nl_emit_pairs.  ``ws`` holds
@@ -191,9 +201,10 @@ int tsx_md_baoa(double* x, double* p, const double* f, const double* m, const in
  * on pos_in, the closing B(dt/2) of the current step when ``end_kick``, and,
  * when ``do_next``, the opening half of the next step (B A O A for Langevin,
  * B A(dt) for NVE) written to pos_out (which must differ from pos_in); else
- * pos_out = pos_in.  Atoms farther than ``half_skin`` from ``xref`` set
- * *flag_set = 1; *flag_clear is reset to 0.  ``step`` is a device counter
- * giving the Philox step of the next step's noise.  w_atom may be NULL. */
+ * pos_out = pos_in.  An atom of system s farther than ``half_skin`` from its
+ * ``xref`` sets flag_set[s] = 1 (the rebuild gate of the next step).
+ * ``step`` is a device counter giving the Philox step of the next step's
+ * noise.  w_atom may be NULL. */
 int tsx_md_fused_step(int32_t kind, const double* params, int32_t n_species,
                       const double* pos_in, double* pos_out, const double* xref,
                       const double* cell_col, const int32_t* sys_start,
@@ -203,13 +214,14 @@ int tsx_md_fused_step(int32_t kind, const double* params, int32_t n_species,
                       double* e_atom, double* w_atom, double dt, int32_t langevin,
                       const double* kT, const double* gamma, const double* noise,
                       uint64_t seed, const int64_t* step, int32_t end_kick, int32_t do_next,
-                      double half_skin, int32_t* flag_set, int32_t* flag_clear,
-                      void* stream);
+                      double half_skin, int32_t* flag_set, void* stream);
 
 /* Per-system energy [S] = sum of e_atom (one workgroup per system, fixed
- * order); when ``advance``, *step += 1 afterwards. */
+ * order); gate_clear[s] = 0 when given (the step's rebuild gates, read by the
+ * list build before the fused step); when ``advance``, *step += 1. */
 int tsx_md_reduce(const double* e_atom, const int32_t* sys_start, int32_t n_systems,
-                  double* energy, int64_t* step, int32_t advance, void* stream);
+                  double* energy, int64_t* step, int32_t advance, int32_t* gate_clear,
+                  void* stream);
 
 /* Copy n_doubles of pos into xref when *gate != 0 (or gate is NULL) and count
  * the copies in *counter (may be NULL): the displacement reference of the

@@ -149,27 +149,32 @@ def test_engine_rebuilds_track_per_step_path(dev):
     del ref
 
 
-def test_engine_nve_batch_equals_single_bitwise(dev):
+def test_engine_batch_equals_single_bitwise_with_rebuilds(dev):
+    """Per-system rebuild gates: each system's list is rebuilt on its own
+    schedule, so a batched run is bit-identical to running each system alone,
+    rebuilds included (reference: test_integrators.py:1104-1154)."""
     st = tsa.workloads.cu_batch(3, device=dev, seed=26)
     model = tsa.MorseModel(**MORSE, cutoff=6.0, device=dev)
     st.rng = 3
-    md = tsa.nve_init(state=st, model=model, kT=300.0 * tsa.MetalUnits.temperature)
+    md = tsa.nve_init(state=st, model=model, kT=1500.0 * tsa.MetalUnits.temperature)
     dt = 0.002 * tsa.MetalUnits.time
-    # a skin wide enough that no rebuild happens keeps every neighbour row
-    # identical between the batched and the single-system runs
-    eng = tsa.BatchedMD(model, md, dt=dt, langevin=False, skin=1.0)
-    eng.run(30)
-    assert eng.n_rebuilds == 0
+    eng = tsa.BatchedMD(model, md, dt=dt, langevin=False, skin=0.2)
+    eng.run(40)
+    assert eng.n_rebuilds > 0
     full = eng.state()
     a = 0
+    total = 0
     for k, single in enumerate(md.split()):
-        e1 = tsa.BatchedMD(model, single, dt=dt, langevin=False, skin=1.0)
-        e1.run(30)
+        e1 = tsa.BatchedMD(model, single, dt=dt, langevin=False, skin=0.2)
+        e1.run(40)
+        total += e1.n_rebuilds
         s1 = e1.state()
         n = single.n_atoms
         assert torch.equal(s1.positions, full.positions[a : a + n])
+        assert torch.equal(s1.momenta, full.momenta[a : a + n])
         assert torch.equal(s1.energy[0], full.energy[k])
         a += n
+    assert total == eng.n_rebuilds
 
 
 def test_engine_nve_energy_conservation(dev):

@@ -33,7 +33,7 @@ _SIGS = {
     "tsx_nl_ws_bytes": (L, [L, I, I]),
     "tsx_nl_build": (
         I,
-        [P, I, P, P, I, P, P, L, I, D, I, I, I, I, I, I, L, I, P, P, P, P, P, P, P],
+        [P, I, P, P, I, P, P, L, I, D, I, I, I, I, I, I, L, I, I, P, P, P, P, P, P, P, P, P],
     ),
     "tsx_scan_ws_bytes": (L, [L]),
     "tsx_nl_count_pairs": (I, [P, L, P, P, P]),
@@ -53,10 +53,10 @@ _SIGS = {
         I,
         [
             I, P, I, P, P, P, P, P, P, P, P, P, L, I, I, L, D,  # pair part
-            P, P, P, P, P, D, I, P, P, P, U, P, I, I, D, P, P, P,  # step part
+            P, P, P, P, P, D, I, P, P, P, U, P, I, I, D, P, P,  # step part
         ],
     ),
-    "tsx_md_reduce": (I, [P, P, I, P, P, I, P]),
+    "tsx_md_reduce": (I, [P, P, I, P, P, I, P, P]),
     "tsx_copy_ref": (I, [P, P, L, P, P, P]),
 }
 

@@ -108,8 +108,7 @@ struct FusedArgs {
   int end_kick;
   int do_next;
   double half_skin2;
-  int32_t* flag_set;
-  int32_t* flag_clear;
+  int32_t* flag_set;  // [S] rebuild request of each system for the next step
 };
 
 template <int KIND, int TPA, bool LANGEVIN, bool VIRIAL>
@@ -117,7 +116,6 @@ __global__ __launch_bounds__(256) void k_md_fused(FusedArgs A) {
   __shared__ PairP sp[MAX_SPECIES_MD * MAX_SPECIES_MD];
   for (int k = threadIdx.x; k < A.ns * A.ns; k += blockDim.x)
     sp[k] = make_pair_params(KIND, A.params[3 * k], A.params[3 * k + 1], A.params[3 * k + 2]);
-  if (A.flag_clear && blockIdx.x == 0 && threadIdx.x == 0) *A.flag_clear = 0;
   __syncthreads();
   const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t t = gid / TPA;
@@ -168,7 +166,7 @@ __global__ __launch_bounds__(256) void k_md_fused(FusedArgs A) {
     if (A.xref && A.flag_set) {
       const double4 r = reinterpret_cast<const double4*>(A.xref)[i];
       const double ex = xo.x - r.x, ey = xo.y - r.y, ez = xo.z - r.z;
-      if (ex * ex + ey * ey + ez * ez > A.half_skin2) *A.flag_set = 1;
+      if (ex * ex + ey * ey + ez * ez > A.half_skin2) A.flag_set[s] = 1;
     }
   }
   reinterpret_cast<double4*>(A.pos_out)[i] = xo;
@@ -183,11 +181,13 @@ __global__ __launch_bounds__(256) void k_md_fused(FusedArgs A) {
     for (int c = 0; c < 6; ++c) A.w_atom[6 * (int64_t)i + c] = a.w[c];
 }
 
-// per-system energy sums (fixed order) + advance the device step counter
+// per-system energy sums (fixed order), clear this step's rebuild gates and
+// advance the device step counter
 __global__ __launch_bounds__(256) void k_md_reduce(const double* __restrict__ e_atom,
                                                    const int32_t* __restrict__ sys_start,
                                                    double* __restrict__ energy,
-                                                   int64_t* __restrict__ step, int advance) {
+                                                   int64_t* __restrict__ step, int advance,
+                                                   int32_t* __restrict__ gate_clear) {
   __shared__ double sh[4];
   const int s = blockIdx.x;
   const int a = sys_start[s], b = sys_start[s + 1];
@@ -196,6 +196,7 @@ __global__ __launch_bounds__(256) void k_md_reduce(const double* __restrict__ e_
   e = block_sum(e, sh);
   if (threadIdx.x == 0) {
     energy[s] = e;
+    if (gate_clear) gate_clear[s] = 0;
     if (advance && s == 0) step[0] += 1;
   }
 }
@@ -288,8 +289,7 @@ int tsx_md_fused_step(int32_t kind, const double* params, int32_t n_species,
                       double* e_atom, double* w_atom, double dt, int32_t langevin,
                       const double* kT, const double* gamma, const double* noise,
                       uint64_t seed, const int64_t* step, int32_t end_kick, int32_t do_next,
-                      double half_skin, int32_t* flag_set, int32_t* flag_clear,
-                      void* stream) {
+                      double half_skin, int32_t* flag_set, void* stream) {
   TSX_REQUIRE(kind >= TSX_LJ && kind <= TSX_SOFT_SPHERE, "unknown pair kind %d", kind);
   TSX_REQUIRE(n_species >= 1 && n_species <= MAX_SPECIES_MD, "n_species out of range");
   TSX_REQUIRE(pos_in != pos_out, "pos_in and pos_out must be different buffers");
@@ -327,7 +327,6 @@ int tsx_md_fused_step(int32_t kind, const double* params, int32_t n_species,
   a.do_next = do_next;
   a.half_skin2 = half_skin * half_skin;
   a.flag_set = flag_set;
-  a.flag_clear = flag_clear;
   hipStream_t st = (hipStream_t)stream;
   const int nblk = blocks_for(n_atoms * tpa, 256);
   const bool vir = w_atom != nullptr;
@@ -344,11 +343,12 @@ int tsx_md_fused_step(int32_t kind, const double* params, int32_t n_species,
 }
 
 int tsx_md_reduce(const double* e_atom, const int32_t* sys_start, int32_t n_systems,
-                  double* energy, int64_t* step, int32_t advance, void* stream) {
+                  double* energy, int64_t* step, int32_t advance, int32_t* gate_clear,
+                  void* stream) {
   TSX_REQUIRE(n_systems > 0, "n_systems must be positive");
   TSX_REQUIRE(!advance || step, "step counter required to advance");
   k_md_reduce<<<n_systems, 256, 0, (hipStream_t)stream>>>(e_atom, sys_start, energy, step,
-                                                           advance);
+                                                           advance, gate_clear);
   TSX_CHECK_LAUNCH();
   return TSX_OK;
 }

@@ -2,18 +2,23 @@
 //
 // Replaces the reference's padded, host-synchronising linked-cell search
 // (torch_sim/transforms.py:1009-1373, torch_sim/neighbors/torch_nl.py:156-215)
-// with per-system fractional grids:
-//   setup   : one workgroup computes each system's inverse cell, plane
-//             spacings, grid size nb_k and stencil half-width m_k, and a scan
-//             of bin counts into per-system bin offsets.
-//   bin     : one thread per atom, fractional coordinate -> bin, atomic count.
-//   scan    : exclusive scan of the bin counts (wave64 shuffles + LDS).
-//   scatter : atoms -> bin order; one-bin systems are written in atom order.
-//   sort    : each multi-atom bin is sorted by atom index (deterministic).
-//   wrap    : wrapped FP64 coordinates (transforms.py:160-210) in bin order.
-//   fill    : one thread per atom walks its stencil, FP64 distance test on
-//             wrapped coordinates (torch_nl.py:87), unwrapped-frame shift
-//             (torch_nl.py:214), ELL write with overflow detection.
+// with per-system fractional grids.  Every stage after ``setup`` can be gated
+// per system, so a Verlet-skin engine rebuilds only the systems whose atoms
+// moved more than half the skin:
+//   setup     : one workgroup computes each system's inverse cell, plane
+//               spacings, grid size nb_k, stencil half-width m_k and bin
+//               offsets; a grid kernel maps every bin to its system.
+//   bin       : one thread per atom, fractional coordinate -> bin, atomic count.
+//   scan      : one workgroup per system scans its bin counts.
+//   scatter   : atoms -> bin order (one-bin systems keep atom order).
+//   sort+wrap : one wave per bin ranks its atoms by index in LDS
+//               (deterministic order) and writes the wrapped FP64 coordinates
+//               (transforms.py:160-210) in bin order.
+//   fill      : one workgroup per bin stages the stencil's candidate atoms in
+//               LDS; each thread owns one atom of the bin and keeps the images
+//               with FP64 |wr_i - wr_j - S.L|^2 < r_list^2 (torch_nl.py:87),
+//               unwrapped-frame shift (torch_nl.py:214), ELL write with
+//               overflow detection.
 #include "tsx_common.h"
 
 #include <cstdarg>
@@ -44,6 +49,7 @@ struct NLWs {
   int32_t* meta;
   int32_t* bin_count;  // bin_cap + 1
   int32_t* bin_start;  // bin_cap + 1
+  int32_t* bin_sys;    // bin_cap + 1
   int32_t* atom_bin;   // N
   int32_t* slot;       // N
   double* wr;          // N x 4 (wrapped xyz, packed wrap shift)
@@ -65,6 +71,7 @@ static size_t ws_layout(int64_t n, int32_t s, int32_t bin_cap, NLWs* w, char* ba
   void* p_meta = take(sizeof(int32_t) * META);
   void* p_bc = take(sizeof(int32_t) * nb);
   void* p_bs = take(sizeof(int32_t) * nb);
+  void* p_bsys = take(sizeof(int32_t) * nb);
   void* p_ab = take(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
   void* p_sl = take(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
   void* p_wr = take(sizeof(double) * 4 * (size_t)(n > 0 ? n : 1));
@@ -75,6 +82,7 @@ static size_t ws_layout(int64_t n, int32_t s, int32_t bin_cap, NLWs* w, char* ba
     w->meta = (int32_t*)p_meta;
     w->bin_count = (int32_t*)p_bc;
     w->bin_start = (int32_t*)p_bs;
+    w->bin_sys = (int32_t*)p_bsys;
     w->atom_bin = (int32_t*)p_ab;
     w->slot = (int32_t*)p_sl;
     w->wr = (double*)p_wr;
@@ -240,25 +248,6 @@ __device__ __forceinline__ int bin_of(const WrapOut& o, const int32_t* gr) {
   return gr[6] + (b[0] * gr[1] + b[1]) * gr[2] + b[2];
 }
 
-__global__ void k_bin(const double* __restrict__ pos, int ps, const int32_t* __restrict__ atom_sys,
-                      int64_t n, const double* __restrict__ geom, const int32_t* __restrict__ grid,
-                      const int32_t* __restrict__ meta, int bin_cap,
-                      int32_t* __restrict__ bin_count, int32_t* __restrict__ atom_bin,
-                      int32_t* __restrict__ slot, const int32_t* __restrict__ gate) {
-  if (gate && *gate == 0) return;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (meta[0] > bin_cap) return;
-  const int s = atom_sys[i];
-  const int32_t* gr = grid + (size_t)GRID * s;
-  const WrapOut o = wrap_atom(pos + ps * i, geom + (size_t)GEOM * s, gr[7]);
-  const int b = bin_of(o, gr);
-  atom_bin[i] = b;
-  slot[i] = atomicAdd(&bin_count[b], 1);
-}
-
-// Exclusive scan of int32 -> T over [0, n] (one extra element = total),
-// single workgroup of 1024 threads; each thread owns a contiguous chunk.
 template <typename Tin, typename T>
 __global__ void k_scan_1block(const Tin* __restrict__ in, int64_t n, T* __restrict__ out, const int32_t* __restrict__ gate) {
   if (gate && *gate == 0) return;
@@ -332,61 +321,184 @@ __global__ void k_scan_final(const int32_t* __restrict__ in, int64_t n,
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_THREADS - 1) out[n] = run;
 }
 
-__global__ void k_scatter(int64_t n, const int32_t* __restrict__ atom_sys,
-                          const int32_t* __restrict__ sys_start, const int32_t* __restrict__ grid,
-                          const int32_t* __restrict__ meta, int bin_cap,
-                          const int32_t* __restrict__ atom_bin, const int32_t* __restrict__ slot,
-                          const int32_t* __restrict__ bin_start, int32_t* __restrict__ order, const int32_t* __restrict__ gate) {
-  if (gate && *gate == 0) return;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (meta[0] > bin_cap) return;
-  const int s = atom_sys[i];
-  const int32_t* gr = grid + (size_t)GRID * s;
-  const int single = gr[0] * gr[1] * gr[2] == 1;
-  // one-bin systems keep atom order; otherwise the atomic slot, sorted later
-  const int64_t dst = single ? (int64_t)i : (int64_t)bin_start[atom_bin[i]] + slot[i];
-  (void)sys_start;
-  order[dst] = (int32_t)i;
-}
 
-__global__ void k_sort_bins(const int32_t* __restrict__ meta, int bin_cap,
-                            const int32_t* __restrict__ bin_start, int32_t* __restrict__ order, const int32_t* __restrict__ gate) {
-  if (gate && *gate == 0) return;
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// ------------------------------------------------------------ bin -> system
+__global__ void k_bin_sys(const int32_t* __restrict__ grid, const int32_t* __restrict__ meta,
+                          int n_sys, int bin_cap, int32_t* __restrict__ bin_sys) {
   const int total = meta[0];
-  if (total > bin_cap || b >= total) return;
-  const int lo = bin_start[b], hi = bin_start[b + 1];
-  for (int k = lo + 1; k < hi; ++k) {
-    const int v = order[k];
-    int p = k - 1;
-    while (p >= lo && order[p] > v) {
-      order[p + 1] = order[p];
-      --p;
+  if (total > bin_cap) return;
+  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < total;
+       b += (int64_t)gridDim.x * blockDim.x) {
+    int lo = 0, hi = n_sys - 1;  // last system with bin_off <= b
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (grid[(size_t)GRID * mid + 6] <= b)
+        lo = mid;
+      else
+        hi = mid - 1;
     }
-    order[p + 1] = v;
+    bin_sys[b] = lo;
   }
 }
 
-__global__ void k_wrap_sorted(const double* __restrict__ pos, int ps,
-                              const int32_t* __restrict__ atom_sys, int64_t n,
-                              const double* __restrict__ geom, const int32_t* __restrict__ grid,
-                              const int32_t* __restrict__ meta, int bin_cap,
-                              const int32_t* __restrict__ order, double* __restrict__ wr, const int32_t* __restrict__ gate) {
-  if (gate && *gate == 0) return;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
+__device__ __forceinline__ bool gated_off(const int32_t* __restrict__ gate, int s) {
+  return gate != nullptr && gate[s] == 0;
+}
+
+// -------------------------------------------------------------------- bin
+// One thread per atom: bin by wrapped fractional coordinate, atomic slot, and
+// (engine) the displacement reference copy of the position.
+__global__ void k_bin(const double* __restrict__ pos, int ps, const int32_t* __restrict__ atom_sys,
+                      int64_t n, const double* __restrict__ geom, const int32_t* __restrict__ grid,
+                      const int32_t* __restrict__ meta, int bin_cap,
+                      int32_t* __restrict__ bin_count, int32_t* __restrict__ atom_bin,
+                      int32_t* __restrict__ slot, const int32_t* __restrict__ gate,
+                      double* __restrict__ xref) {
   if (meta[0] > bin_cap) return;
-  const int i = order[t];
-  const int s = atom_sys[i];
-  const WrapOut o = wrap_atom(pos + (int64_t)ps * i, geom + (size_t)GEOM * s,
-                              grid[(size_t)GRID * s + 7]);
-  double4 v;
-  v.x = o.wr[0];
-  v.y = o.wr[1];
-  v.z = o.wr[2];
-  v.w = pack_w(o.w[0], o.w[1], o.w[2]);
-  reinterpret_cast<double4*>(wr)[t] = v;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int s = atom_sys[i];
+    if (gated_off(gate, s)) continue;
+    const int32_t* gr = grid + (size_t)GRID * s;
+    const double* r = pos + ps * i;
+    const WrapOut o = wrap_atom(r, geom + (size_t)GEOM * s, gr[7]);
+    const int b = bin_of(o, gr);
+    atom_bin[i] = b;
+    slot[i] = atomicAdd(&bin_count[b], 1);
+    if (xref) {
+      double4 v;
+      v.x = r[0];
+      v.y = r[1];
+      v.z = r[2];
+      v.w = ps == 4 ? r[3] : 0.0;
+      reinterpret_cast<double4*>(xref)[i] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------- per-system scan
+// One workgroup (256) per system: exclusive scan of its bin counts offset by
+// the system's first atom; bin_start[off + nbins] = sys_start[s + 1].
+__global__ __launch_bounds__(256) void k_scan_sys(
+    const int32_t* __restrict__ grid, const int32_t* __restrict__ meta, int bin_cap,
+    const int32_t* __restrict__ sys_start, const int32_t* __restrict__ bin_count,
+    int32_t* __restrict__ bin_start, const int32_t* __restrict__ gate,
+    int32_t* __restrict__ counter) {
+  __shared__ int32_t sh[256];
+  __shared__ int32_t carry;
+  const int s = blockIdx.x;
+  if (meta[0] > bin_cap || gated_off(gate, s)) return;
+  const int32_t* gr = grid + (size_t)GRID * s;
+  const int off = gr[6], nb = gr[0] * gr[1] * gr[2];
+  if (threadIdx.x == 0) carry = sys_start[s];
+  __syncthreads();
+  for (int c0 = 0; c0 < nb; c0 += 256) {
+    const int c = c0 + threadIdx.x;
+    const int v = c < nb ? bin_count[off + c] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+      const int t = threadIdx.x >= (unsigned)d ? sh[threadIdx.x - d] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (c < nb) bin_start[off + c] = carry + sh[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 255) carry += sh[255];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    bin_start[off + nb] = sys_start[s + 1];
+    if (counter) atomicAdd(counter, 1);
+  }
+}
+
+__global__ void k_scatter(int64_t n, const int32_t* __restrict__ atom_sys,
+                          const int32_t* __restrict__ grid, const int32_t* __restrict__ meta,
+                          int bin_cap, const int32_t* __restrict__ atom_bin,
+                          const int32_t* __restrict__ slot, const int32_t* __restrict__ bin_start,
+                          int32_t* __restrict__ order, const int32_t* __restrict__ gate) {
+  if (meta[0] > bin_cap) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int s = atom_sys[i];
+    if (gated_off(gate, s)) continue;
+    const int32_t* gr = grid + (size_t)GRID * s;
+    const bool single = gr[0] * gr[1] * gr[2] == 1;
+    // one-bin systems keep atom order; otherwise the atomic slot, sorted later
+    const int64_t dst = single ? i : (int64_t)bin_start[atom_bin[i]] + slot[i];
+    order[dst] = (int32_t)i;
+  }
+}
+
+// ---------------------------------------------------------- sort + wrap
+// One wave (block of 64) per bin: order the bin's atoms by index (rank sort
+// in LDS; atom indices are unique), reset the bin counter for the next build,
+// and write the wrapped FP64 coordinates in bin order.
+constexpr int SORT_CAP = 2048;
+
+__global__ __launch_bounds__(64) void k_sort_wrap(
+    const double* __restrict__ pos, int ps, const double* __restrict__ geom,
+    const int32_t* __restrict__ grid, const int32_t* __restrict__ meta, int bin_cap,
+    const int32_t* __restrict__ bin_sys, const int32_t* __restrict__ bin_start,
+    int32_t* __restrict__ bin_count, int32_t* __restrict__ order, double* __restrict__ wr,
+    const int32_t* __restrict__ gate) {
+  __shared__ int32_t keys[SORT_CAP];
+  __shared__ int unsorted;
+  const int total = meta[0];
+  if (total > bin_cap) return;
+  const int lane = threadIdx.x;
+  for (int b = blockIdx.x; b < total; b += gridDim.x) {
+    const int s = bin_sys[b];
+    if (gated_off(gate, s)) continue;
+    const int lo = bin_start[b], hi = bin_start[b + 1], n = hi - lo;
+    if (n > 1 && n <= SORT_CAP) {
+      if (lane == 0) unsorted = 0;
+      __syncthreads();
+      for (int k = lane; k < n; k += 64) keys[k] = order[lo + k];
+      __syncthreads();
+      for (int k = lane; k + 1 < n; k += 64)
+        if (keys[k] > keys[k + 1]) unsorted = 1;
+      __syncthreads();
+      if (unsorted) {
+        for (int k = lane; k < n; k += 64) {
+          const int v = keys[k];
+          int rank = 0;
+          for (int q = 0; q < n; ++q) rank += keys[q] < v;
+          order[lo + rank] = v;
+        }
+      }
+      __threadfence_block();
+      __syncthreads();
+    } else if (n > SORT_CAP && lane == 0) {
+      for (int k = lo + 1; k < hi; ++k) {  // rare: very large multi-bin bins
+        const int v = order[k];
+        int p = k - 1;
+        while (p >= lo && order[p] > v) {
+          order[p + 1] = order[p];
+          --p;
+        }
+        order[p + 1] = v;
+      }
+      __threadfence_block();
+    }
+    __syncthreads();
+    if (lane == 0) bin_count[b] = 0;
+    const double* g = geom + (size_t)GEOM * s;
+    const int flags = grid[(size_t)GRID * s + 7];
+    for (int k = lo + lane; k < hi; k += 64) {
+      const int i = order[k];
+      const WrapOut o = wrap_atom(pos + (int64_t)ps * i, g, flags);
+      double4 v;
+      v.x = o.wr[0];
+      v.y = o.wr[1];
+      v.z = o.wr[2];
+      v.w = pack_w(o.w[0], o.w[1], o.w[2]);
+      reinterpret_cast<double4*>(wr)[k] = v;
+    }
+    __syncthreads();
+  }
 }
 
 __device__ __forceinline__ int floordiv(int a, int b) {
@@ -411,103 +523,160 @@ __device__ __forceinline__ int64_t ell_index(int64_t t, int n, int tpa, int64_t 
   return (int64_t)(n / tpa) * (ld * tpa) + t * tpa + (n % tpa);
 }
 
-__global__ void k_fill(const int32_t* __restrict__ atom_sys, const int32_t* __restrict__ sys_start,
-                       int64_t n_atoms, const double* __restrict__ geom,
-                       const int32_t* __restrict__ grid, const int32_t* __restrict__ meta,
-                       int bin_cap, const int32_t* __restrict__ bin_start,
-                       const int32_t* __restrict__ atom_bin,
-                       const int32_t* __restrict__ order, const double* __restrict__ wr,
-                       double rl2, int self_int, int half, int jbits, int tpa, int kmax,
-                       int64_t ld, uint32_t* __restrict__ nbr, int32_t* __restrict__ count,
-                       int32_t* __restrict__ status, const int32_t* __restrict__ gate) {
-  if (gate && *gate == 0) return;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_atoms) return;
-  if (meta[0] > bin_cap) {
-    count[t] = 0;
-    return;
-  }
-  const int i = order[t];
-  const int s = atom_sys[i];
-  const int base = sys_start[s];
-  const double* g = geom + (size_t)GEOM * s;
-  const int32_t* gr = grid + (size_t)GRID * s;
-  const double4 wi = reinterpret_cast<const double4*>(wr)[t];
-  int w_i[3];
-  unpack_w(wi.w, w_i);
-  int bi[3];
-  {
-    const int local = atom_bin[i] - gr[6];
-    bi[2] = local % gr[2];
-    bi[1] = (local / gr[2]) % gr[1];
-    bi[0] = local / (gr[1] * gr[2]);
-  }
+
+// ------------------------------------------------------------------- fill
+// One workgroup (one wave) per bin.  The stencil images of the bin (bin +
+// lattice shift S, m_k bins each way) are enumerated in LDS; their atoms are
+// staged in LDS in chunks and every thread tests them against its own atom.
+// Neighbour order within a row: image order (dx, dy, dz nested), then atom
+// index within the image's bin -- deterministic.
+constexpr int FILL_T = 64;
+constexpr int FILL_CH = 256;   // staged candidates per chunk
+constexpr int FILL_IMG = 128;  // images per batch
+
+__global__ __launch_bounds__(FILL_T) void k_fill(
+    const int32_t* __restrict__ sys_start, const double* __restrict__ geom,
+    const int32_t* __restrict__ grid, const int32_t* __restrict__ meta, int bin_cap,
+    const int32_t* __restrict__ bin_sys, const int32_t* __restrict__ bin_start,
+    const int32_t* __restrict__ order, const double* __restrict__ wr, double rl2, int self_int,
+    int half, int jbits, int tpa, int kmax, int64_t ld, uint32_t* __restrict__ nbr,
+    int32_t* __restrict__ count, int32_t* __restrict__ status,
+    const int32_t* __restrict__ gate) {
+  __shared__ double4 cpos[FILL_CH];
+  __shared__ int32_t ck[FILL_CH];
+  __shared__ int32_t cimg[FILL_CH];
+  __shared__ double ics[FILL_IMG][3];
+  __shared__ int32_t isx[FILL_IMG], isy[FILL_IMG], isz[FILL_IMG];
+  __shared__ int32_t ibeg[FILL_IMG], ipre[FILL_IMG + 1];
+  const int total = meta[0];
+  if (total > bin_cap) return;
+  const int tid = threadIdx.x;
   const int sbits = (32 - jbits) / 3;
   const int bias = 1 << (sbits - 1);
   const uint32_t fmask = (1u << sbits) - 1u;
-  int cnt = 0;
-  bool shift_ovf = false;
-  for (int dx = -gr[3]; dx <= gr[3]; ++dx) {
-    const int cx = bi[0] + dx;
-    const int sx = floordiv(cx, gr[0]);
-    const int bx = cx - sx * gr[0];
-    for (int dy = -gr[4]; dy <= gr[4]; ++dy) {
-      const int cy = bi[1] + dy;
-      const int sy = floordiv(cy, gr[1]);
-      const int by = cy - sy * gr[1];
-      for (int dz = -gr[5]; dz <= gr[5]; ++dz) {
-        const int cz = bi[2] + dz;
-        const int sz = floordiv(cz, gr[2]);
-        const int bz = cz - sz * gr[2];
-        const int b = gr[6] + (bx * gr[1] + by) * gr[2] + bz;
-        // S . L in the reference einsum order
-        const double csx = (double)sx * g[0] + (double)sy * g[3] + (double)sz * g[6];
-        const double csy = (double)sx * g[1] + (double)sy * g[4] + (double)sz * g[7];
-        const double csz = (double)sx * g[2] + (double)sy * g[5] + (double)sz * g[8];
-        const bool zero_shift = (sx | sy | sz) == 0;
-        const int k0 = bin_start[b], k1 = bin_start[b + 1];
-        for (int k = k0; k < k1; ++k) {
-          const double4 wj = reinterpret_cast<const double4*>(wr)[k];
-          const double d2 = dist2_ref(wi.x, wi.y, wi.z, wj.x, wj.y, wj.z, csx, csy, csz);
-          if (!(d2 < rl2)) continue;
-          const int j = order[k];
-          if (j == i && zero_shift && !self_int) continue;
-          int w_j[3];
-          unpack_w(wj.w, w_j);
-          const int ux = sx + w_i[0] - w_j[0];
-          const int uy = sy + w_i[1] - w_j[1];
-          const int uz = sz + w_i[2] - w_j[2];
-          if (half) {
-            bool keep;
-            if (i != j) {
-              keep = i < j;
-            } else {
-              const int first = ux != 0 ? ux : (uy != 0 ? uy : uz);
-              keep = first > 0;
-            }
-            if (!keep) continue;
-          }
-          if (ux < -bias || ux >= bias || uy < -bias || uy >= bias || uz < -bias || uz >= bias) {
-            shift_ovf = true;
-            continue;
-          }
-          const uint32_t word = (uint32_t)(j - base) |
-                                ((((uint32_t)(ux + bias) & fmask) |
-                                  (((uint32_t)(uy + bias) & fmask) << sbits) |
-                                  (((uint32_t)(uz + bias) & fmask) << (2 * sbits)))
-                                 << jbits);
-          if (cnt < kmax) nbr[ell_index(t, cnt, tpa, ld)] = word;
-          ++cnt;
+  for (int b = blockIdx.x; b < total; b += gridDim.x) {
+    const int s = bin_sys[b];
+    if (gated_off(gate, s)) continue;
+    const int c0 = bin_start[b], c1 = bin_start[b + 1];
+    if (c1 == c0) continue;
+    const double* g = geom + (size_t)GEOM * s;
+    const int32_t* gr = grid + (size_t)GRID * s;
+    const int base = sys_start[s];
+    const int local = b - gr[6];
+    const int bz0 = local % gr[2], by0 = (local / gr[2]) % gr[1], bx0 = local / (gr[1] * gr[2]);
+    const int wx = 2 * gr[3] + 1, wy = 2 * gr[4] + 1, wz = 2 * gr[5] + 1;
+    const int n_img = wx * wy * wz;
+    for (int r0 = c0; r0 < c1; r0 += FILL_T) {  // rounds of center atoms
+      const int t = r0 + tid;
+      const bool own = t < c1;
+      double4 wi = make_double4(0.0, 0.0, 0.0, 0.0);
+      int i = -1, w_i[3] = {0, 0, 0};
+      if (own) {
+        wi = reinterpret_cast<const double4*>(wr)[t];
+        i = order[t];
+        unpack_w(wi.w, w_i);
+      }
+      int cnt = 0;
+      bool shift_ovf = false;
+      for (int ib0 = 0; ib0 < n_img; ib0 += FILL_IMG) {
+        const int nib = min(FILL_IMG, n_img - ib0);
+        __syncthreads();
+        for (int q = tid; q < nib; q += FILL_T) {
+          const int im = ib0 + q;
+          const int dz = im % wz - gr[5];
+          const int dy = (im / wz) % wy - gr[4];
+          const int dx = im / (wy * wz) - gr[3];
+          const int cx = bx0 + dx, cy = by0 + dy, cz = bz0 + dz;
+          const int sx = floordiv(cx, gr[0]), sy = floordiv(cy, gr[1]), sz = floordiv(cz, gr[2]);
+          const int nbin = gr[6] + ((cx - sx * gr[0]) * gr[1] + (cy - sy * gr[1])) * gr[2] +
+                           (cz - sz * gr[2]);
+          // S . L in the reference einsum order (transforms.py:563-565)
+          ics[q][0] = (double)sx * g[0] + (double)sy * g[3] + (double)sz * g[6];
+          ics[q][1] = (double)sx * g[1] + (double)sy * g[4] + (double)sz * g[7];
+          ics[q][2] = (double)sx * g[2] + (double)sy * g[5] + (double)sz * g[8];
+          isx[q] = sx;
+          isy[q] = sy;
+          isz[q] = sz;
+          ibeg[q] = bin_start[nbin];
+          ipre[q + 1] = bin_start[nbin + 1] - bin_start[nbin];
         }
+        __syncthreads();
+        if (tid == 0) {
+          ipre[0] = 0;
+          for (int q = 0; q < nib; ++q) ipre[q + 1] += ipre[q];
+        }
+        __syncthreads();
+        const int ncand = ipre[nib];
+        for (int k0 = 0; k0 < ncand; k0 += FILL_CH) {
+          const int nk = min(FILL_CH, ncand - k0);
+          __syncthreads();
+          for (int c = tid; c < nk; c += FILL_T) {
+            const int gc = k0 + c;
+            int lo = 0, hi = nib - 1;  // image holding candidate gc
+            while (lo < hi) {
+              const int mid = (lo + hi + 1) >> 1;
+              if (ipre[mid] <= gc)
+                lo = mid;
+              else
+                hi = mid - 1;
+            }
+            const int k = ibeg[lo] + (gc - ipre[lo]);
+            cpos[c] = reinterpret_cast<const double4*>(wr)[k];
+            ck[c] = k;
+            cimg[c] = lo;
+          }
+          __syncthreads();
+          if (own) {
+            for (int c = 0; c < nk; ++c) {
+              const double4 wj = cpos[c];
+              const int q = cimg[c];
+              const double d2 =
+                  dist2_ref(wi.x, wi.y, wi.z, wj.x, wj.y, wj.z, ics[q][0], ics[q][1], ics[q][2]);
+              if (!(d2 < rl2)) continue;
+              const int j = order[ck[c]];
+              const int sx = isx[q], sy = isy[q], sz = isz[q];
+              if (j == i && (sx | sy | sz) == 0 && !self_int) continue;
+              int w_j[3];
+              unpack_w(wj.w, w_j);
+              const int ux = sx + w_i[0] - w_j[0];
+              const int uy = sy + w_i[1] - w_j[1];
+              const int uz = sz + w_i[2] - w_j[2];
+              if (half) {
+                bool keep;
+                if (i != j) {
+                  keep = i < j;
+                } else {
+                  const int first = ux != 0 ? ux : (uy != 0 ? uy : uz);
+                  keep = first > 0;
+                }
+                if (!keep) continue;
+              }
+              if (ux < -bias || ux >= bias || uy < -bias || uy >= bias || uz < -bias ||
+                  uz >= bias) {
+                shift_ovf = true;
+                continue;
+              }
+              const uint32_t word = (uint32_t)(j - base) |
+                                    ((((uint32_t)(ux + bias) & fmask) |
+                                      (((uint32_t)(uy + bias) & fmask) << sbits) |
+                                      (((uint32_t)(uz + bias) & fmask) << (2 * sbits)))
+                                     << jbits);
+              if (cnt < kmax) nbr[ell_index(t, cnt, tpa, ld)] = word;
+              ++cnt;
+            }
+          }
+        }
+      }
+      if (own) {
+        if (cnt > kmax) {
+          atomicOr(status, TSX_STATUS_NL_OVERFLOW);
+          cnt = kmax;
+        }
+        if (shift_ovf) atomicOr(status, TSX_STATUS_SHIFT_OVERFLOW);
+        count[t] = cnt;
       }
     }
   }
-  if (cnt > kmax) {
-    atomicOr(status, TSX_STATUS_NL_OVERFLOW);
-    cnt = kmax;
-  }
-  if (shift_ovf) atomicOr(status, TSX_STATUS_SHIFT_OVERFLOW);
-  count[t] = cnt;
 }
 
 __global__ void k_emit(const uint32_t* __restrict__ nbr, const int32_t* __restrict__ count,
@@ -570,9 +739,9 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
                  const uint8_t* pbc, int32_t pbc_per_system, const int32_t* sys_start,
                  const int32_t* atom_sys, int64_t n_atoms, int32_t n_systems, double r_list,
                  int32_t bins_per_rlist, int32_t self_interaction, int32_t half, int32_t jbits,
-                 int32_t tpa, int32_t kmax, int64_t ld, int32_t bin_cap, void* ws,
-                 uint32_t* nbr, int32_t* count, int32_t* order, int32_t* status,
-                 const int32_t* gate, void* stream) {
+                 int32_t tpa, int32_t kmax, int64_t ld, int32_t bin_cap, int32_t setup, void* ws,
+                 uint32_t* nbr, int32_t* count, int32_t* order, double* xref, int32_t* status,
+                 const int32_t* gate, int32_t* counter, void* stream) {
   TSX_REQUIRE(n_systems > 0, "n_systems must be positive");
   TSX_REQUIRE(n_atoms >= 0 && n_atoms < (1ll << 31), "n_atoms out of range");
   TSX_REQUIRE(pos_stride == 3 || pos_stride == 4, "pos_stride must be 3 or 4");
@@ -580,36 +749,44 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
   TSX_REQUIRE(tpa == 1 || tpa == 2 || tpa == 4 || tpa == 8 || tpa == 16,
               "tpa must be a power of two <= 16");
   TSX_REQUIRE(ld >= n_atoms, "ld must be >= n_atoms");
-  TSX_REQUIRE(kmax >= 0, "kmax must be >= 0");
+  TSX_REQUIRE(kmax >= 0 && kmax % tpa == 0, "kmax must be a non-negative multiple of tpa");
   TSX_REQUIRE(r_list > 0.0, "r_list must be positive");
   TSX_REQUIRE(bins_per_rlist >= 1 && bins_per_rlist <= 4, "bins_per_rlist in [1,4]");
   TSX_REQUIRE(bin_cap >= 1, "bin_cap must be >= 1");
   hipStream_t st = (hipStream_t)stream;
   NLWs w;
   ws_layout(n_atoms, n_systems, bin_cap, &w, (char*)ws);
-  const int zero_blocks = blocks_for((int64_t)bin_cap + 1, 256);
-  k_setup<<<zero_blocks < 1024 ? zero_blocks : 1024, 1024, 0, st>>>(
-      cell_col, pbc, pbc_per_system, n_systems, r_list, bins_per_rlist, bin_cap, 1, w.geom,
-      w.grid, w.meta, status, w.bin_count, gate);
-  TSX_CHECK_LAUNCH();
+  if (setup) {
+    const int zb = blocks_for((int64_t)bin_cap + 1, 1024);
+    k_setup<<<zb < 1024 ? zb : 1024, 1024, 0, st>>>(
+        cell_col, pbc, pbc_per_system, n_systems, r_list, bins_per_rlist, bin_cap, 1, w.geom,
+        w.grid, w.meta, status, w.bin_count, nullptr);
+    TSX_CHECK_LAUNCH();
+    const int bb = blocks_for(bin_cap, 256);
+    k_bin_sys<<<bb < 1024 ? bb : 1024, 256, 0, st>>>(w.grid, w.meta, n_systems, bin_cap,
+                                                     w.bin_sys);
+    TSX_CHECK_LAUNCH();
+  }
   if (n_atoms == 0) return TSX_OK;
-  const int nb = blocks_for(n_atoms, 256);
-  k_bin<<<nb, 256, 0, st>>>(pos, pos_stride, atom_sys, n_atoms, w.geom, w.grid, w.meta,
-                            bin_cap, w.bin_count, w.atom_bin, w.slot, gate);
+  const int na = blocks_for(n_atoms, 256);
+  const int grid_atoms = na < 2048 ? na : 2048;
+  const int grid_bins = bin_cap < 4096 ? bin_cap : 4096;
+  k_bin<<<grid_atoms, 256, 0, st>>>(pos, pos_stride, atom_sys, n_atoms, w.geom, w.grid, w.meta,
+                                    bin_cap, w.bin_count, w.atom_bin, w.slot, gate, xref);
   TSX_CHECK_LAUNCH();
-  k_scan_1block<int32_t, int32_t><<<1, 1024, 0, st>>>(w.bin_count, (int64_t)bin_cap, w.bin_start, gate);
+  k_scan_sys<<<n_systems, 256, 0, st>>>(w.grid, w.meta, bin_cap, sys_start, w.bin_count,
+                                        w.bin_start, gate, counter);
   TSX_CHECK_LAUNCH();
-  k_scatter<<<nb, 256, 0, st>>>(n_atoms, atom_sys, sys_start, w.grid, w.meta, bin_cap,
-                                w.atom_bin, w.slot, w.bin_start, order, gate);
+  k_scatter<<<grid_atoms, 256, 0, st>>>(n_atoms, atom_sys, w.grid, w.meta, bin_cap, w.atom_bin,
+                                        w.slot, w.bin_start, order, gate);
   TSX_CHECK_LAUNCH();
-  k_sort_bins<<<blocks_for(bin_cap, 256), 256, 0, st>>>(w.meta, bin_cap, w.bin_start, order, gate);
+  k_sort_wrap<<<grid_bins, 64, 0, st>>>(pos, pos_stride, w.geom, w.grid, w.meta, bin_cap,
+                                        w.bin_sys, w.bin_start, w.bin_count, order, w.wr, gate);
   TSX_CHECK_LAUNCH();
-  k_wrap_sorted<<<nb, 256, 0, st>>>(pos, pos_stride, atom_sys, n_atoms, w.geom, w.grid, w.meta,
-                                    bin_cap, order, w.wr, gate);
-  TSX_CHECK_LAUNCH();
-  k_fill<<<nb, 256, 0, st>>>(atom_sys, sys_start, n_atoms, w.geom, w.grid, w.meta, bin_cap,
-                             w.bin_start, w.atom_bin, order, w.wr, r_list * r_list, self_interaction, half,
-                             jbits, tpa, kmax, ld, nbr, count, status, gate);
+  k_fill<<<grid_bins, FILL_T, 0, st>>>(sys_start, w.geom, w.grid, w.meta, bin_cap, w.bin_sys,
+                                       w.bin_start, order, w.wr, r_list * r_list,
+                                       self_interaction, half, jbits, tpa, kmax, ld, nbr, count,
+                                       status, gate);
   TSX_CHECK_LAUNCH();
   return TSX_OK;
 }

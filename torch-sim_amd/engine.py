@@ -117,7 +117,8 @@ class BatchedMD:
         g_t = torch.as_tensor(g, dtype=torch.float64, device=dev)
         self.gamma = (g_t.expand(s) if g_t.ndim == 0 else g_t).contiguous().clone()
         self.step_ctr = torch.ones(1, dtype=torch.int64, device=dev)
-        self.flags = torch.zeros(2, dtype=torch.int32, device=dev)
+        # per-system rebuild gates, one row per step parity
+        self.flags = torch.zeros((2, s), dtype=torch.int32, device=dev)
         self.rebuilds = torch.zeros(1, dtype=torch.int32, device=dev)
         self.nl = NeighborList(model.cutoff + self.skin, tpa=tpa, bins_per_rlist=bins_per_rlist)
         self.cur = 0
@@ -128,7 +129,7 @@ class BatchedMD:
 
     # ---------------------------------------------------------- kernels
     def _gate(self, q: int) -> torch.Tensor:
-        return self.flags[q : q + 1]
+        return self.flags[q]
 
     def _fused(self, q: int, *, end_kick: bool, do_next: bool, noise=None) -> None:
         nl = self.nl
@@ -142,38 +143,34 @@ class BatchedMD:
             ptr(self.p), ptr(self.m), ptr(self.f), ptr(self.e_atom), None, self.dt,
             int(self.langevin), ptr(self.kT), ptr(self.gamma), ptr(noise), self.seed,
             ptr(self.step_ctr), int(end_kick), int(do_next), 0.5 * self.skin,
-            ptr(self._gate(1 - q)), ptr(self._gate(q)), stream(),
+            ptr(self._gate(1 - q)), stream(),
         )
 
-    def _reduce(self, advance: bool = True) -> None:
+    def _reduce(self, q: int) -> None:
         call(
             "tsx_md_reduce", ptr(self.e_atom), ptr(self.sysidx.sys_start), self.n_systems,
-            ptr(self.energy), ptr(self.step_ctr), int(advance), stream(),
+            ptr(self.energy), ptr(self.step_ctr), 1, ptr(self._gate(q)), stream(),
         )
 
-    def _rebuild(self, q: int, gated: bool) -> None:
-        gate = self._gate(q) if gated else None
+    def _rebuild(self, q: int) -> None:
+        """Gated list rebuild of the systems flagged in gate row q (cells are
+        constant during MD, so the geometry from the first build is reused)."""
         self.nl.enqueue(
-            self.pos[q], 4, self.cell, self.pbc, self.pbc_per_system, self.sysidx, gate=gate
-        )
-        call(
-            "tsx_copy_ref", ptr(self.pos[q]), ptr(self.xref), 4 * self.n_atoms, ptr(gate),
-            ptr(self.rebuilds), stream(),
+            self.pos[q], 4, self.cell, self.pbc, self.pbc_per_system, self.sysidx,
+            setup=False, gate=self._gate(q), xref=self.xref, counter=self.rebuilds,
         )
 
     def _step(self, q: int, do_next: bool, noise=None) -> None:
-        self._rebuild(q, gated=True)
+        self._rebuild(q)
         self._fused(q, end_kick=True, do_next=do_next, noise=noise)
-        self._reduce(advance=True)
+        self._reduce(q)
 
     # ------------------------------------------------------------ control
     def initialize(self) -> None:
         """Host-checked list build on the initial positions."""
         if self.n_atoms:
-            self.nl.build(self.pos[0], self.cell, self._pbc_bool, self.sysidx, pos_stride=4)
-            call(
-                "tsx_copy_ref", ptr(self.pos[0]), ptr(self.xref), 4 * self.n_atoms, None, None,
-                stream(),
+            self.nl.build(
+                self.pos[0], self.cell, self._pbc_bool, self.sysidx, pos_stride=4, xref=self.xref
             )
         self.flags.zero_()
         self.cur = 0
@@ -184,7 +181,7 @@ class BatchedMD:
         step boundary) and the opening half of the next step."""
         q = self.cur
         self._fused(q, end_kick=False, do_next=True, noise=self._next_noise())
-        self._reduce(advance=True)
+        self._reduce(q)
         self.cur = 1 - q
 
     def _next_noise(self):
@@ -264,8 +261,10 @@ class BatchedMD:
                 bins = self.nl.bin_cap * 2 if st & STATUS_BIN_OVERFLOW else self.nl.bin_cap
                 self.nl.ensure(self.n_atoms, self.n_systems, kmax, bins, self.nl.jbits, self.device)
                 self._graph = None
+                # full rebuild (with geometry) at the restored positions
+                self.nl.build(self.pos[self.cur], self.cell, self._pbc_bool, self.sysidx,
+                              pos_stride=4, xref=self.xref)
                 self.flags.zero_()
-                self.flags[self.cur] = 1  # rebuild at the restored positions
                 continue
             if st:
                 raise _lib.TsxError(f"engine status {st:#x} (shift overflow or singular cell)")
@@ -300,6 +299,7 @@ class BatchedMD:
 
     @property
     def n_rebuilds(self) -> int:
+        """Number of per-system list rebuilds run by the gated builds."""
         return int(self.rebuilds.item())
 
     @property

@@ -182,17 +182,24 @@ class NeighborList:
         pbc_u8: torch.Tensor,
         pbc_per_system: bool,
         sysidx: SystemIndex,
+        *,
+        setup: bool = True,
         gate: torch.Tensor | None = None,
+        xref: torch.Tensor | None = None,
+        counter: torch.Tensor | None = None,
     ) -> None:
-        """Enqueue a build on the current stream (capturable, no host sync)."""
+        """Enqueue a build on the current stream (capturable, no host sync).
+
+        ``gate`` [S] int32 restricts the rebuild to flagged systems; ``setup``
+        recomputes the per-system geometry (needed whenever a cell changed)."""
         call(
             "tsx_nl_build",
             ptr(pos), pos_stride, ptr(cell_col), ptr(pbc_u8), int(pbc_per_system),
             ptr(sysidx.sys_start), ptr(sysidx.atom_sys), sysidx.n_atoms, sysidx.n_systems,
             self.r_list, self.bins_per_rlist, int(self.self_interaction), int(self.half),
-            self.jbits, self.tpa, self.kmax, self.ld, self.bin_cap, ptr(self.ws),
-            ptr(self.nbr), ptr(self.count), ptr(self.order), ptr(self.status), ptr(gate),
-            stream(),
+            self.jbits, self.tpa, self.kmax, self.ld, self.bin_cap, int(setup), ptr(self.ws),
+            ptr(self.nbr), ptr(self.count), ptr(self.order), ptr(xref), ptr(self.status),
+            ptr(gate), ptr(counter), stream(),
         )
 
     def build(
@@ -202,7 +209,8 @@ class NeighborList:
         pbc: torch.Tensor,
         sysidx: SystemIndex,
         pos_stride: int = 3,
-        max_retries: int = 6,
+        max_retries: int = 12,
+        xref: torch.Tensor | None = None,
     ) -> None:
         """Build with host-side capacity planning and overflow retries."""
         _lib.require_gpu(pos, "positions")
@@ -222,7 +230,7 @@ class NeighborList:
         self.jbits = jbits
         for _ in range(max_retries):
             self.status.zero_()
-            self.enqueue(pos, pos_stride, cell_col, pbc_u8, pbc_per_system, sysidx)
+            self.enqueue(pos, pos_stride, cell_col, pbc_u8, pbc_per_system, sysidx, xref=xref)
             st = int(self.status.item())
             if st & STATUS_SINGULAR_CELL:
                 raise _lib.TsxError("periodic system with a singular cell")
@@ -234,7 +242,7 @@ class NeighborList:
             if not st & (STATUS_NL_OVERFLOW | STATUS_BIN_OVERFLOW):
                 return
             kmax = self.kmax * 2 if st & STATUS_NL_OVERFLOW else self.kmax
-            bins = self.bin_cap * 2 if st & STATUS_BIN_OVERFLOW else self.bin_cap
+            bins = self.bin_cap * 4 if st & STATUS_BIN_OVERFLOW else self.bin_cap
             self.ensure(sysidx.n_atoms, sysidx.n_systems, kmax, bins, jbits, pos.device)
         raise _lib.TsxError("neighbor list capacity could not be resolved")
 
