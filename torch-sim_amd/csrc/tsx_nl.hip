@@ -49,7 +49,6 @@ struct NLWs {
   int32_t* meta;
   int32_t* bin_count;  // bin_cap + 1
   int32_t* bin_start;  // bin_cap + 1
-  int32_t* bin_sys;    // bin_cap + 1
   int32_t* atom_bin;   // N
   int32_t* slot;       // N
   double* wr;          // N x 4 (wrapped xyz, packed wrap shift)
@@ -71,7 +70,6 @@ static size_t ws_layout(int64_t n, int32_t s, int32_t bin_cap, NLWs* w, char* ba
   void* p_meta = take(sizeof(int32_t) * META);
   void* p_bc = take(sizeof(int32_t) * nb);
   void* p_bs = take(sizeof(int32_t) * nb);
-  void* p_bsys = take(sizeof(int32_t) * nb);
   void* p_ab = take(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
   void* p_sl = take(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
   void* p_wr = take(sizeof(double) * 4 * (size_t)(n > 0 ? n : 1));
@@ -82,7 +80,6 @@ static size_t ws_layout(int64_t n, int32_t s, int32_t bin_cap, NLWs* w, char* ba
     w->meta = (int32_t*)p_meta;
     w->bin_count = (int32_t*)p_bc;
     w->bin_start = (int32_t*)p_bs;
-    w->bin_sys = (int32_t*)p_bsys;
     w->atom_bin = (int32_t*)p_ab;
     w->slot = (int32_t*)p_sl;
     w->wr = (double*)p_wr;
@@ -322,46 +319,31 @@ __global__ void k_scan_final(const int32_t* __restrict__ in, int64_t n,
 }
 
 
-// ------------------------------------------------------------ bin -> system
-__global__ void k_bin_sys(const int32_t* __restrict__ grid, const int32_t* __restrict__ meta,
-                          int n_sys, int bin_cap, int32_t* __restrict__ bin_sys) {
-  const int total = meta[0];
-  if (total > bin_cap) return;
-  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < total;
-       b += (int64_t)gridDim.x * blockDim.x) {
-    int lo = 0, hi = n_sys - 1;  // last system with bin_off <= b
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (grid[(size_t)GRID * mid + 6] <= b)
-        lo = mid;
-      else
-        hi = mid - 1;
-    }
-    bin_sys[b] = lo;
-  }
-}
-
 __device__ __forceinline__ bool gated_off(const int32_t* __restrict__ gate, int s) {
   return gate != nullptr && gate[s] == 0;
 }
 
+// The per-system kernels below run on a grid of n_sys * B blocks: block
+// (s, part) handles system s, striding its atoms / bins by B, so systems whose
+// gate is off cost one load per block.
+
 // -------------------------------------------------------------------- bin
-// One thread per atom: bin by wrapped fractional coordinate, atomic slot, and
-// (engine) the displacement reference copy of the position.
-__global__ void k_bin(const double* __restrict__ pos, int ps, const int32_t* __restrict__ atom_sys,
-                      int64_t n, const double* __restrict__ geom, const int32_t* __restrict__ grid,
-                      const int32_t* __restrict__ meta, int bin_cap,
-                      int32_t* __restrict__ bin_count, int32_t* __restrict__ atom_bin,
-                      int32_t* __restrict__ slot, const int32_t* __restrict__ gate,
-                      double* __restrict__ xref) {
-  if (meta[0] > bin_cap) return;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int s = atom_sys[i];
-    if (gated_off(gate, s)) continue;
-    const int32_t* gr = grid + (size_t)GRID * s;
-    const double* r = pos + ps * i;
-    const WrapOut o = wrap_atom(r, geom + (size_t)GEOM * s, gr[7]);
+// Bin by wrapped fractional coordinate, atomic slot, and (engine) the
+// displacement reference copy of the position.
+__global__ __launch_bounds__(256) void k_bin(
+    const double* __restrict__ pos, int ps, const int32_t* __restrict__ sys_start, int n_sys,
+    int B, const double* __restrict__ geom, const int32_t* __restrict__ grid,
+    const int32_t* __restrict__ meta, int bin_cap, int32_t* __restrict__ bin_count,
+    int32_t* __restrict__ atom_bin, int32_t* __restrict__ slot,
+    const int32_t* __restrict__ gate, double* __restrict__ xref) {
+  const int s = blockIdx.x / B, part = blockIdx.x % B;
+  if (s >= n_sys || meta[0] > bin_cap || gated_off(gate, s)) return;
+  const int32_t* gr = grid + (size_t)GRID * s;
+  const double* g = geom + (size_t)GEOM * s;
+  const int e = sys_start[s + 1];
+  for (int i = sys_start[s] + part * blockDim.x + threadIdx.x; i < e; i += B * blockDim.x) {
+    const double* r = pos + (int64_t)ps * i;
+    const WrapOut o = wrap_atom(r, g, gr[7]);
     const int b = bin_of(o, gr);
     atom_bin[i] = b;
     slot[i] = atomicAdd(&bin_count[b], 1);
@@ -414,21 +396,20 @@ __global__ __launch_bounds__(256) void k_scan_sys(
   }
 }
 
-__global__ void k_scatter(int64_t n, const int32_t* __restrict__ atom_sys,
-                          const int32_t* __restrict__ grid, const int32_t* __restrict__ meta,
-                          int bin_cap, const int32_t* __restrict__ atom_bin,
-                          const int32_t* __restrict__ slot, const int32_t* __restrict__ bin_start,
-                          int32_t* __restrict__ order, const int32_t* __restrict__ gate) {
-  if (meta[0] > bin_cap) return;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int s = atom_sys[i];
-    if (gated_off(gate, s)) continue;
-    const int32_t* gr = grid + (size_t)GRID * s;
-    const bool single = gr[0] * gr[1] * gr[2] == 1;
+__global__ __launch_bounds__(256) void k_scatter(
+    const int32_t* __restrict__ sys_start, int n_sys, int B, const int32_t* __restrict__ grid,
+    const int32_t* __restrict__ meta, int bin_cap, const int32_t* __restrict__ atom_bin,
+    const int32_t* __restrict__ slot, const int32_t* __restrict__ bin_start,
+    int32_t* __restrict__ order, const int32_t* __restrict__ gate) {
+  const int s = blockIdx.x / B, part = blockIdx.x % B;
+  if (s >= n_sys || meta[0] > bin_cap || gated_off(gate, s)) return;
+  const int32_t* gr = grid + (size_t)GRID * s;
+  const bool single = gr[0] * gr[1] * gr[2] == 1;
+  const int e = sys_start[s + 1];
+  for (int i = sys_start[s] + part * blockDim.x + threadIdx.x; i < e; i += B * blockDim.x) {
     // one-bin systems keep atom order; otherwise the atomic slot, sorted later
-    const int64_t dst = single ? i : (int64_t)bin_start[atom_bin[i]] + slot[i];
-    order[dst] = (int32_t)i;
+    const int dst = single ? i : bin_start[atom_bin[i]] + slot[i];
+    order[dst] = i;
   }
 }
 
@@ -439,19 +420,19 @@ __global__ void k_scatter(int64_t n, const int32_t* __restrict__ atom_sys,
 constexpr int SORT_CAP = 2048;
 
 __global__ __launch_bounds__(64) void k_sort_wrap(
-    const double* __restrict__ pos, int ps, const double* __restrict__ geom,
+    const double* __restrict__ pos, int ps, int n_sys, int B, const double* __restrict__ geom,
     const int32_t* __restrict__ grid, const int32_t* __restrict__ meta, int bin_cap,
-    const int32_t* __restrict__ bin_sys, const int32_t* __restrict__ bin_start,
-    int32_t* __restrict__ bin_count, int32_t* __restrict__ order, double* __restrict__ wr,
-    const int32_t* __restrict__ gate) {
+    const int32_t* __restrict__ bin_start, int32_t* __restrict__ bin_count,
+    int32_t* __restrict__ order, double* __restrict__ wr, const int32_t* __restrict__ gate) {
   __shared__ int32_t keys[SORT_CAP];
   __shared__ int unsorted;
-  const int total = meta[0];
-  if (total > bin_cap) return;
+  const int s = blockIdx.x / B, part = blockIdx.x % B;
+  if (s >= n_sys || meta[0] > bin_cap || gated_off(gate, s)) return;
   const int lane = threadIdx.x;
-  for (int b = blockIdx.x; b < total; b += gridDim.x) {
-    const int s = bin_sys[b];
-    if (gated_off(gate, s)) continue;
+  const int32_t* gr = grid + (size_t)GRID * s;
+  const double* g = geom + (size_t)GEOM * s;
+  const int off = gr[6], nbins = gr[0] * gr[1] * gr[2];
+  for (int b = off + part; b < off + nbins; b += B) {
     const int lo = bin_start[b], hi = bin_start[b + 1], n = hi - lo;
     if (n > 1 && n <= SORT_CAP) {
       if (lane == 0) unsorted = 0;
@@ -485,11 +466,9 @@ __global__ __launch_bounds__(64) void k_sort_wrap(
     }
     __syncthreads();
     if (lane == 0) bin_count[b] = 0;
-    const double* g = geom + (size_t)GEOM * s;
-    const int flags = grid[(size_t)GRID * s + 7];
     for (int k = lo + lane; k < hi; k += 64) {
       const int i = order[k];
-      const WrapOut o = wrap_atom(pos + (int64_t)ps * i, g, flags);
+      const WrapOut o = wrap_atom(pos + (int64_t)ps * i, g, gr[7]);
       double4 v;
       v.x = o.wr[0];
       v.y = o.wr[1];
@@ -525,49 +504,57 @@ __device__ __forceinline__ int64_t ell_index(int64_t t, int n, int tpa, int64_t 
 
 
 // ------------------------------------------------------------------- fill
-// One workgroup (one wave) per bin.  The stencil images of the bin (bin +
-// lattice shift S, m_k bins each way) are enumerated in LDS; their atoms are
-// staged in LDS in chunks and every thread tests them against its own atom.
-// Neighbour order within a row: image order (dx, dy, dz nested), then atom
-// index within the image's bin -- deterministic.
+// One wave per bin (block (s, part) takes the system's bins with stride B).
+// The stencil images of the bin (bin + lattice shift S, m_k bins each way)
+// are enumerated in LDS and their atoms staged in LDS in chunks.  L lanes
+// (a power of two, L * atoms-per-round = 64) share one centre atom: lane
+// ``sub`` tests candidates sub, sub + L, ...; a wave ballot orders the matches
+// so every row lists its neighbours in candidate order -- image order
+// (dx, dy, dz nested), then atom index -- whatever L is (deterministic).
 constexpr int FILL_T = 64;
 constexpr int FILL_CH = 256;   // staged candidates per chunk
 constexpr int FILL_IMG = 128;  // images per batch
 
 __global__ __launch_bounds__(FILL_T) void k_fill(
-    const int32_t* __restrict__ sys_start, const double* __restrict__ geom,
+    const int32_t* __restrict__ sys_start, int n_sys, int B, const double* __restrict__ geom,
     const int32_t* __restrict__ grid, const int32_t* __restrict__ meta, int bin_cap,
-    const int32_t* __restrict__ bin_sys, const int32_t* __restrict__ bin_start,
-    const int32_t* __restrict__ order, const double* __restrict__ wr, double rl2, int self_int,
-    int half, int jbits, int tpa, int kmax, int64_t ld, uint32_t* __restrict__ nbr,
-    int32_t* __restrict__ count, int32_t* __restrict__ status,
-    const int32_t* __restrict__ gate) {
+    const int32_t* __restrict__ bin_start, const int32_t* __restrict__ order,
+    const double* __restrict__ wr, double rl2, int self_int, int half, int jbits, int tpa,
+    int kmax, int64_t ld, uint32_t* __restrict__ nbr, int32_t* __restrict__ count,
+    int32_t* __restrict__ status, const int32_t* __restrict__ gate) {
   __shared__ double4 cpos[FILL_CH];
   __shared__ int32_t ck[FILL_CH];
   __shared__ int32_t cimg[FILL_CH];
   __shared__ double ics[FILL_IMG][3];
   __shared__ int32_t isx[FILL_IMG], isy[FILL_IMG], isz[FILL_IMG];
   __shared__ int32_t ibeg[FILL_IMG], ipre[FILL_IMG + 1];
-  const int total = meta[0];
-  if (total > bin_cap) return;
+  const int s = blockIdx.x / B, part = blockIdx.x % B;
+  if (s >= n_sys || meta[0] > bin_cap || gated_off(gate, s)) return;
   const int tid = threadIdx.x;
   const int sbits = (32 - jbits) / 3;
   const int bias = 1 << (sbits - 1);
   const uint32_t fmask = (1u << sbits) - 1u;
-  for (int b = blockIdx.x; b < total; b += gridDim.x) {
-    const int s = bin_sys[b];
-    if (gated_off(gate, s)) continue;
+  const double* g = geom + (size_t)GEOM * s;
+  const int32_t* gr = grid + (size_t)GRID * s;
+  const int base = sys_start[s];
+  const int off = gr[6], nbins = gr[0] * gr[1] * gr[2];
+  const int wx = 2 * gr[3] + 1, wy = 2 * gr[4] + 1, wz = 2 * gr[5] + 1;
+  const int n_img = wx * wy * wz;
+  for (int b = off + part; b < off + nbins; b += B) {
     const int c0 = bin_start[b], c1 = bin_start[b + 1];
-    if (c1 == c0) continue;
-    const double* g = geom + (size_t)GEOM * s;
-    const int32_t* gr = grid + (size_t)GRID * s;
-    const int base = sys_start[s];
-    const int local = b - gr[6];
+    const int nc = c1 - c0;
+    if (nc == 0) continue;
+    const int local = b - off;
     const int bz0 = local % gr[2], by0 = (local / gr[2]) % gr[1], bx0 = local / (gr[1] * gr[2]);
-    const int wx = 2 * gr[3] + 1, wy = 2 * gr[4] + 1, wz = 2 * gr[5] + 1;
-    const int n_img = wx * wy * wz;
-    for (int r0 = c0; r0 < c1; r0 += FILL_T) {  // rounds of center atoms
-      const int t = r0 + tid;
+    // lanes per centre atom
+    int per = 1;
+    while (per < nc && per < FILL_T) per <<= 1;  // centre atoms per round (pow2)
+    const int L = FILL_T / per;
+    const int slot_a = tid / L, sub = tid % L;
+    const uint64_t gmask = (L == 64) ? ~0ull : (((1ull << L) - 1ull) << (slot_a * L));
+    const uint64_t below = (sub == 0) ? 0ull : (((1ull << sub) - 1ull) << (slot_a * L));
+    for (int r0 = c0; r0 < c1; r0 += per) {  // rounds of centre atoms
+      const int t = r0 + slot_a;
       const bool own = t < c1;
       double4 wi = make_double4(0.0, 0.0, 0.0, 0.0);
       int i = -1, w_i[3] = {0, 0, 0};
@@ -588,7 +575,7 @@ __global__ __launch_bounds__(FILL_T) void k_fill(
           const int dx = im / (wy * wz) - gr[3];
           const int cx = bx0 + dx, cy = by0 + dy, cz = bz0 + dz;
           const int sx = floordiv(cx, gr[0]), sy = floordiv(cy, gr[1]), sz = floordiv(cz, gr[2]);
-          const int nbin = gr[6] + ((cx - sx * gr[0]) * gr[1] + (cy - sy * gr[1])) * gr[2] +
+          const int nbin = off + ((cx - sx * gr[0]) * gr[1] + (cy - sy * gr[1])) * gr[2] +
                            (cz - sz * gr[2]);
           // S . L in the reference einsum order (transforms.py:563-565)
           ics[q][0] = (double)sx * g[0] + (double)sy * g[3] + (double)sz * g[6];
@@ -626,53 +613,60 @@ __global__ __launch_bounds__(FILL_T) void k_fill(
             cimg[c] = lo;
           }
           __syncthreads();
-          if (own) {
-            for (int c = 0; c < nk; ++c) {
+          for (int cb = 0; cb < nk; cb += L) {  // uniform trip count over the wave
+            const int c = cb + sub;
+            bool match = false;
+            uint32_t word = 0;
+            if (own && c < nk) {
               const double4 wj = cpos[c];
               const int q = cimg[c];
               const double d2 =
                   dist2_ref(wi.x, wi.y, wi.z, wj.x, wj.y, wj.z, ics[q][0], ics[q][1], ics[q][2]);
-              if (!(d2 < rl2)) continue;
-              const int j = order[ck[c]];
-              const int sx = isx[q], sy = isy[q], sz = isz[q];
-              if (j == i && (sx | sy | sz) == 0 && !self_int) continue;
-              int w_j[3];
-              unpack_w(wj.w, w_j);
-              const int ux = sx + w_i[0] - w_j[0];
-              const int uy = sy + w_i[1] - w_j[1];
-              const int uz = sz + w_i[2] - w_j[2];
-              if (half) {
-                bool keep;
-                if (i != j) {
-                  keep = i < j;
-                } else {
-                  const int first = ux != 0 ? ux : (uy != 0 ? uy : uz);
-                  keep = first > 0;
+              if (d2 < rl2) {
+                const int j = order[ck[c]];
+                const int sx = isx[q], sy = isy[q], sz = isz[q];
+                match = !(j == i && (sx | sy | sz) == 0 && !self_int);
+                int w_j[3];
+                unpack_w(wj.w, w_j);
+                const int ux = sx + w_i[0] - w_j[0];
+                const int uy = sy + w_i[1] - w_j[1];
+                const int uz = sz + w_i[2] - w_j[2];
+                if (match && half) {
+                  if (i != j) {
+                    match = i < j;
+                  } else {
+                    const int first = ux != 0 ? ux : (uy != 0 ? uy : uz);
+                    match = first > 0;
+                  }
                 }
-                if (!keep) continue;
+                if (match && (ux < -bias || ux >= bias || uy < -bias || uy >= bias ||
+                              uz < -bias || uz >= bias)) {
+                  shift_ovf = true;
+                  match = false;
+                }
+                word = (uint32_t)(j - base) |
+                       ((((uint32_t)(ux + bias) & fmask) | (((uint32_t)(uy + bias) & fmask) << sbits) |
+                         (((uint32_t)(uz + bias) & fmask) << (2 * sbits)))
+                        << jbits);
               }
-              if (ux < -bias || ux >= bias || uy < -bias || uy >= bias || uz < -bias ||
-                  uz >= bias) {
-                shift_ovf = true;
-                continue;
-              }
-              const uint32_t word = (uint32_t)(j - base) |
-                                    ((((uint32_t)(ux + bias) & fmask) |
-                                      (((uint32_t)(uy + bias) & fmask) << sbits) |
-                                      (((uint32_t)(uz + bias) & fmask) << (2 * sbits)))
-                                     << jbits);
-              if (cnt < kmax) nbr[ell_index(t, cnt, tpa, ld)] = word;
-              ++cnt;
             }
+            const uint64_t bal = __ballot(match);
+            if (match) {
+              const int pos_n = cnt + __popcll(bal & below);
+              if (pos_n < kmax) nbr[ell_index(t, pos_n, tpa, ld)] = word;
+            }
+            cnt += __popcll(bal & gmask);
           }
         }
       }
-      if (own) {
+      // every lane of a group holds the same count; lane sub == 0 reports
+      const uint64_t ovf = __ballot(shift_ovf);
+      if (own && sub == 0) {
         if (cnt > kmax) {
           atomicOr(status, TSX_STATUS_NL_OVERFLOW);
           cnt = kmax;
         }
-        if (shift_ovf) atomicOr(status, TSX_STATUS_SHIFT_OVERFLOW);
+        if (ovf & gmask) atomicOr(status, TSX_STATUS_SHIFT_OVERFLOW);
         count[t] = cnt;
       }
     }
@@ -762,31 +756,31 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
         cell_col, pbc, pbc_per_system, n_systems, r_list, bins_per_rlist, bin_cap, 1, w.geom,
         w.grid, w.meta, status, w.bin_count, nullptr);
     TSX_CHECK_LAUNCH();
-    const int bb = blocks_for(bin_cap, 256);
-    k_bin_sys<<<bb < 1024 ? bb : 1024, 256, 0, st>>>(w.grid, w.meta, n_systems, bin_cap,
-                                                     w.bin_sys);
-    TSX_CHECK_LAUNCH();
   }
   if (n_atoms == 0) return TSX_OK;
-  const int na = blocks_for(n_atoms, 256);
-  const int grid_atoms = na < 2048 ? na : 2048;
-  const int grid_bins = bin_cap < 4096 ? bin_cap : 4096;
-  k_bin<<<grid_atoms, 256, 0, st>>>(pos, pos_stride, atom_sys, n_atoms, w.geom, w.grid, w.meta,
-                                    bin_cap, w.bin_count, w.atom_bin, w.slot, gate, xref);
+  // blocks per system: whole-grid parallelism for one large system, a few
+  // blocks each for hundreds of small ones
+  const int b_atoms = n_systems >= 2048 ? 1 : 2048 / n_systems;
+  const int b_bins = n_systems >= 8192 ? 1 : 8192 / n_systems;
+  k_bin<<<n_systems * b_atoms, 256, 0, st>>>(pos, pos_stride, sys_start, n_systems, b_atoms,
+                                             w.geom, w.grid, w.meta, bin_cap, w.bin_count,
+                                             w.atom_bin, w.slot, gate, xref);
   TSX_CHECK_LAUNCH();
   k_scan_sys<<<n_systems, 256, 0, st>>>(w.grid, w.meta, bin_cap, sys_start, w.bin_count,
                                         w.bin_start, gate, counter);
   TSX_CHECK_LAUNCH();
-  k_scatter<<<grid_atoms, 256, 0, st>>>(n_atoms, atom_sys, w.grid, w.meta, bin_cap, w.atom_bin,
-                                        w.slot, w.bin_start, order, gate);
+  k_scatter<<<n_systems * b_atoms, 256, 0, st>>>(sys_start, n_systems, b_atoms, w.grid, w.meta,
+                                                 bin_cap, w.atom_bin, w.slot, w.bin_start, order,
+                                                 gate);
   TSX_CHECK_LAUNCH();
-  k_sort_wrap<<<grid_bins, 64, 0, st>>>(pos, pos_stride, w.geom, w.grid, w.meta, bin_cap,
-                                        w.bin_sys, w.bin_start, w.bin_count, order, w.wr, gate);
+  k_sort_wrap<<<n_systems * b_bins, 64, 0, st>>>(pos, pos_stride, n_systems, b_bins, w.geom,
+                                                 w.grid, w.meta, bin_cap, w.bin_start,
+                                                 w.bin_count, order, w.wr, gate);
   TSX_CHECK_LAUNCH();
-  k_fill<<<grid_bins, FILL_T, 0, st>>>(sys_start, w.geom, w.grid, w.meta, bin_cap, w.bin_sys,
-                                       w.bin_start, order, w.wr, r_list * r_list,
-                                       self_interaction, half, jbits, tpa, kmax, ld, nbr, count,
-                                       status, gate);
+  k_fill<<<n_systems * b_bins, FILL_T, 0, st>>>(sys_start, n_systems, b_bins, w.geom, w.grid,
+                                                w.meta, bin_cap, w.bin_start, order, w.wr,
+                                                r_list * r_list, self_interaction, half, jbits,
+                                                tpa, kmax, ld, nbr, count, status, gate);
   TSX_CHECK_LAUNCH();
   return TSX_OK;
 }

@@ -68,7 +68,7 @@ class BatchedMD:
         langevin: bool = True,
         skin: float = 0.5,
         tpa: int = 4,
-        bins_per_rlist: int = 1,
+        bins_per_rlist: int = 2,
         seed: int = 0,
         use_graph: bool = True,
         chunk_steps: int = 200,
