@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void k_md_fused(FusedArgs A) {
   for (int k = threadIdx.x; k < A.ns * A.ns; k += blockDim.x)
     sp[k] = make_pair_params(KIND, A.params[3 * k], A.params[3 * k + 1], A.params[3 * k + 2]);
   __syncthreads();
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gid = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   const int64_t t = gid / TPA;
   const int lane = (int)(gid % TPA);
   const bool valid = t < A.n;

@@ -414,35 +414,41 @@ __global__ __launch_bounds__(256) void k_scatter(
 }
 
 // ---------------------------------------------------------- sort + wrap
-// One wave (block of 64) per bin: order the bin's atoms by index (rank sort
-// in LDS; atom indices are unique), reset the bin counter for the next build,
-// and write the wrapped FP64 coordinates in bin order.
-constexpr int SORT_CAP = 2048;
+// Four independent waves per block, one bin at a time per wave (wave w of
+// block (s, part) takes bin off + part * 4 + w, then strides by 4 B): order
+// the bin's atoms by index (rank sort in the wave's LDS slice; atom indices
+// are unique), reset the bin counter for the next build, and write the
+// wrapped FP64 coordinates in bin order.
+constexpr int SORT_W = 4;
+constexpr int SORT_CAP = 1024;
 
-__global__ __launch_bounds__(64) void k_sort_wrap(
+__device__ __forceinline__ void wave_sync_sort() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__global__ __launch_bounds__(64 * SORT_W) void k_sort_wrap(
     const double* __restrict__ pos, int ps, int n_sys, int B, const double* __restrict__ geom,
     const int32_t* __restrict__ grid, const int32_t* __restrict__ meta, int bin_cap,
     const int32_t* __restrict__ bin_start, int32_t* __restrict__ bin_count,
     int32_t* __restrict__ order, double* __restrict__ wr, const int32_t* __restrict__ gate) {
-  __shared__ int32_t keys[SORT_CAP];
-  __shared__ int unsorted;
+  __shared__ int32_t keys_all[SORT_W][SORT_CAP];
   const int s = blockIdx.x / B, part = blockIdx.x % B;
   if (s >= n_sys || meta[0] > bin_cap || gated_off(gate, s)) return;
-  const int lane = threadIdx.x;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int32_t* keys = keys_all[wv];
   const int32_t* gr = grid + (size_t)GRID * s;
   const double* g = geom + (size_t)GEOM * s;
   const int off = gr[6], nbins = gr[0] * gr[1] * gr[2];
-  for (int b = off + part; b < off + nbins; b += B) {
+  for (int b = off + part * SORT_W + wv; b < off + nbins; b += B * SORT_W) {
     const int lo = bin_start[b], hi = bin_start[b + 1], n = hi - lo;
     if (n > 1 && n <= SORT_CAP) {
-      if (lane == 0) unsorted = 0;
-      __syncthreads();
       for (int k = lane; k < n; k += 64) keys[k] = order[lo + k];
-      __syncthreads();
-      for (int k = lane; k + 1 < n; k += 64)
-        if (keys[k] > keys[k + 1]) unsorted = 1;
-      __syncthreads();
-      if (unsorted) {
+      wave_sync_sort();
+      bool uns = false;
+      for (int k = lane; k + 1 < n; k += 64) uns |= keys[k] > keys[k + 1];
+      if (__ballot(uns) != 0ull) {
         for (int k = lane; k < n; k += 64) {
           const int v = keys[k];
           int rank = 0;
@@ -451,20 +457,22 @@ __global__ __launch_bounds__(64) void k_sort_wrap(
         }
       }
       __threadfence_block();
-      __syncthreads();
-    } else if (n > SORT_CAP && lane == 0) {
-      for (int k = lo + 1; k < hi; ++k) {  // rare: very large multi-bin bins
-        const int v = order[k];
-        int p = k - 1;
-        while (p >= lo && order[p] > v) {
-          order[p + 1] = order[p];
-          --p;
+      wave_sync_sort();
+    } else if (n > SORT_CAP) {
+      if (lane == 0) {  // rare: very large bins of multi-bin systems
+        for (int k = lo + 1; k < hi; ++k) {
+          const int v = order[k];
+          int p = k - 1;
+          while (p >= lo && order[p] > v) {
+            order[p + 1] = order[p];
+            --p;
+          }
+          order[p + 1] = v;
         }
-        order[p + 1] = v;
       }
       __threadfence_block();
+      wave_sync_sort();
     }
-    __syncthreads();
     if (lane == 0) bin_count[b] = 0;
     for (int k = lo + lane; k < hi; k += 64) {
       const int i = order[k];
@@ -476,7 +484,7 @@ __global__ __launch_bounds__(64) void k_sort_wrap(
       v.w = pack_w(o.w[0], o.w[1], o.w[2]);
       reinterpret_cast<double4*>(wr)[k] = v;
     }
-    __syncthreads();
+    wave_sync_sort();
   }
 }
 
@@ -504,33 +512,60 @@ __device__ __forceinline__ int64_t ell_index(int64_t t, int n, int tpa, int64_t 
 
 
 // ------------------------------------------------------------------- fill
-// One wave per bin (block (s, part) takes the system's bins with stride B).
-// The stencil images of the bin (bin + lattice shift S, m_k bins each way)
-// are enumerated in LDS and their atoms staged in LDS in chunks.  L lanes
-// (a power of two, L * atoms-per-round = 64) share one centre atom: lane
-// ``sub`` tests candidates sub, sub + L, ...; a wave ballot orders the matches
-// so every row lists its neighbours in candidate order -- image order
-// (dx, dy, dz nested), then atom index -- whatever L is (deterministic).
-constexpr int FILL_T = 64;
-constexpr int FILL_CH = 256;   // staged candidates per chunk
+// Four independent waves per block; each wave owns a stream of bins (wave w
+// of block (s, part) takes bin off + part * 4 + w, then strides by 4 B).
+// All LDS traffic of a wave stays in its own LDS slice, so waves synchronise
+// with wave barriers only.  For one bin, the stencil images (bin + lattice
+// shift S, m_k bins each way) are enumerated in LDS and their atoms staged in
+// LDS in chunks; L lanes (L * centre-atoms-per-round = 64) share one centre
+// atom: lane ``sub`` tests candidates sub, sub + L, ...; a wave ballot orders
+// the matches so every row lists its neighbours in candidate order -- image
+// order (dx, dy, dz nested), then atom index -- for any L (deterministic).
+constexpr int FILL_W = 4;      // waves per block
+constexpr int FILL_CH = 128;   // staged candidates per chunk
 constexpr int FILL_IMG = 128;  // images per batch
 
-__global__ __launch_bounds__(FILL_T) void k_fill(
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// inclusive scan of one int per lane over the wave
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(64 * FILL_W) void k_fill(
     const int32_t* __restrict__ sys_start, int n_sys, int B, const double* __restrict__ geom,
     const int32_t* __restrict__ grid, const int32_t* __restrict__ meta, int bin_cap,
     const int32_t* __restrict__ bin_start, const int32_t* __restrict__ order,
     const double* __restrict__ wr, double rl2, int self_int, int half, int jbits, int tpa,
     int kmax, int64_t ld, uint32_t* __restrict__ nbr, int32_t* __restrict__ count,
     int32_t* __restrict__ status, const int32_t* __restrict__ gate) {
-  __shared__ double4 cpos[FILL_CH];
-  __shared__ int32_t ck[FILL_CH];
-  __shared__ int32_t cimg[FILL_CH];
-  __shared__ double ics[FILL_IMG][3];
-  __shared__ int32_t isx[FILL_IMG], isy[FILL_IMG], isz[FILL_IMG];
-  __shared__ int32_t ibeg[FILL_IMG], ipre[FILL_IMG + 1];
+  __shared__ double4 cpos_all[FILL_W][FILL_CH];
+  __shared__ int32_t cj_all[FILL_W][FILL_CH];
+  __shared__ int32_t cimg_all[FILL_W][FILL_CH];
+  __shared__ double ics_all[FILL_W][FILL_IMG][3];
+  __shared__ int32_t is_all[FILL_W][FILL_IMG][3];
+  __shared__ int32_t ibeg_all[FILL_W][FILL_IMG];
+  __shared__ int32_t ipre_all[FILL_W][FILL_IMG + 1];
   const int s = blockIdx.x / B, part = blockIdx.x % B;
   if (s >= n_sys || meta[0] > bin_cap || gated_off(gate, s)) return;
-  const int tid = threadIdx.x;
+  const int wv = threadIdx.x >> 6, tid = threadIdx.x & 63;
+  double4* cpos = cpos_all[wv];
+  int32_t* cj = cj_all[wv];
+  int32_t* cimg = cimg_all[wv];
+  double(*ics)[3] = ics_all[wv];
+  int32_t(*isv)[3] = is_all[wv];
+  int32_t* ibeg = ibeg_all[wv];
+  int32_t* ipre = ipre_all[wv];
   const int sbits = (32 - jbits) / 3;
   const int bias = 1 << (sbits - 1);
   const uint32_t fmask = (1u << sbits) - 1u;
@@ -540,16 +575,15 @@ __global__ __launch_bounds__(FILL_T) void k_fill(
   const int off = gr[6], nbins = gr[0] * gr[1] * gr[2];
   const int wx = 2 * gr[3] + 1, wy = 2 * gr[4] + 1, wz = 2 * gr[5] + 1;
   const int n_img = wx * wy * wz;
-  for (int b = off + part; b < off + nbins; b += B) {
+  for (int b = off + part * FILL_W + wv; b < off + nbins; b += B * FILL_W) {
     const int c0 = bin_start[b], c1 = bin_start[b + 1];
     const int nc = c1 - c0;
     if (nc == 0) continue;
     const int local = b - off;
     const int bz0 = local % gr[2], by0 = (local / gr[2]) % gr[1], bx0 = local / (gr[1] * gr[2]);
-    // lanes per centre atom
-    int per = 1;
-    while (per < nc && per < FILL_T) per <<= 1;  // centre atoms per round (pow2)
-    const int L = FILL_T / per;
+    int per = 1;  // centre atoms per round (power of two)
+    while (per < nc && per < 64) per <<= 1;
+    const int L = 64 / per;
     const int slot_a = tid / L, sub = tid % L;
     const uint64_t gmask = (L == 64) ? ~0ull : (((1ull << L) - 1ull) << (slot_a * L));
     const uint64_t below = (sub == 0) ? 0ull : (((1ull << sub) - 1ull) << (slot_a * L));
@@ -567,8 +601,11 @@ __global__ __launch_bounds__(FILL_T) void k_fill(
       bool shift_ovf = false;
       for (int ib0 = 0; ib0 < n_img; ib0 += FILL_IMG) {
         const int nib = min(FILL_IMG, n_img - ib0);
-        __syncthreads();
-        for (int q = tid; q < nib; q += FILL_T) {
+        wave_sync();
+        int lsum = 0;
+        for (int h = 0; h < 2; ++h) {  // images 2 * tid, 2 * tid + 1
+          const int q = 2 * tid + h;
+          if (q >= nib) continue;
           const int im = ib0 + q;
           const int dz = im % wz - gr[5];
           const int dy = (im / wz) % wy - gr[4];
@@ -581,23 +618,30 @@ __global__ __launch_bounds__(FILL_T) void k_fill(
           ics[q][0] = (double)sx * g[0] + (double)sy * g[3] + (double)sz * g[6];
           ics[q][1] = (double)sx * g[1] + (double)sy * g[4] + (double)sz * g[7];
           ics[q][2] = (double)sx * g[2] + (double)sy * g[5] + (double)sz * g[8];
-          isx[q] = sx;
-          isy[q] = sy;
-          isz[q] = sz;
-          ibeg[q] = bin_start[nbin];
-          ipre[q + 1] = bin_start[nbin + 1] - bin_start[nbin];
+          isv[q][0] = sx;
+          isv[q][1] = sy;
+          isv[q][2] = sz;
+          const int kb = bin_start[nbin], ke = bin_start[nbin + 1];
+          ibeg[q] = kb;
+          ipre[q + 1] = ke - kb;
+          lsum += ke - kb;
         }
-        __syncthreads();
-        if (tid == 0) {
-          ipre[0] = 0;
-          for (int q = 0; q < nib; ++q) ipre[q + 1] += ipre[q];
+        // exclusive prefix of the image sizes: lane-pair sums, wave scan
+        const int incl = wave_incl_scan(lsum);
+        const int excl = incl - lsum;
+        wave_sync();
+        if (2 * tid < nib) {
+          const int n0 = ipre[2 * tid + 1];
+          ipre[2 * tid] = excl;
+          if (2 * tid + 1 < nib) ipre[2 * tid + 1] = excl + n0;
         }
-        __syncthreads();
+        if (tid == 63) ipre[nib] = incl;
+        wave_sync();
         const int ncand = ipre[nib];
         for (int k0 = 0; k0 < ncand; k0 += FILL_CH) {
           const int nk = min(FILL_CH, ncand - k0);
-          __syncthreads();
-          for (int c = tid; c < nk; c += FILL_T) {
+          wave_sync();
+          for (int c = tid; c < nk; c += 64) {
             const int gc = k0 + c;
             int lo = 0, hi = nib - 1;  // image holding candidate gc
             while (lo < hi) {
@@ -609,10 +653,10 @@ __global__ __launch_bounds__(FILL_T) void k_fill(
             }
             const int k = ibeg[lo] + (gc - ipre[lo]);
             cpos[c] = reinterpret_cast<const double4*>(wr)[k];
-            ck[c] = k;
+            cj[c] = order[k];
             cimg[c] = lo;
           }
-          __syncthreads();
+          wave_sync();
           for (int cb = 0; cb < nk; cb += L) {  // uniform trip count over the wave
             const int c = cb + sub;
             bool match = false;
@@ -623,8 +667,8 @@ __global__ __launch_bounds__(FILL_T) void k_fill(
               const double d2 =
                   dist2_ref(wi.x, wi.y, wi.z, wj.x, wj.y, wj.z, ics[q][0], ics[q][1], ics[q][2]);
               if (d2 < rl2) {
-                const int j = order[ck[c]];
-                const int sx = isx[q], sy = isy[q], sz = isz[q];
+                const int j = cj[c];
+                const int sx = isv[q][0], sy = isv[q][1], sz = isv[q][2];
                 match = !(j == i && (sx | sy | sz) == 0 && !self_int);
                 int w_j[3];
                 unpack_w(wj.w, w_j);
@@ -773,11 +817,11 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
                                                  bin_cap, w.atom_bin, w.slot, w.bin_start, order,
                                                  gate);
   TSX_CHECK_LAUNCH();
-  k_sort_wrap<<<n_systems * b_bins, 64, 0, st>>>(pos, pos_stride, n_systems, b_bins, w.geom,
+  k_sort_wrap<<<n_systems * b_bins, 64 * SORT_W, 0, st>>>(pos, pos_stride, n_systems, b_bins, w.geom,
                                                  w.grid, w.meta, bin_cap, w.bin_start,
                                                  w.bin_count, order, w.wr, gate);
   TSX_CHECK_LAUNCH();
-  k_fill<<<n_systems * b_bins, FILL_T, 0, st>>>(sys_start, n_systems, b_bins, w.geom, w.grid,
+  k_fill<<<n_systems * b_bins, 64 * FILL_W, 0, st>>>(sys_start, n_systems, b_bins, w.geom, w.grid,
                                                 w.meta, bin_cap, w.bin_start, order, w.wr,
                                                 r_list * r_list, self_interaction, half, jbits,
                                                 tpa, kmax, ld, nbr, count, status, gate);

@@ -11,12 +11,13 @@ namespace tsx {
 //   Morse:       a = sigma,   b = eps,    c = alpha,  d = 2 eps alpha
 //   soft sphere: a = sigma,   b = eps,    c = alpha,  d = eps / sigma
 struct PairP {
-  float a, b, c, d;
+  float a, b, c, d, e;
 };
 
 __host__ __device__ inline PairP make_pair_params(int kind, double sigma, double eps,
                                                   double alpha) {
   PairP p;
+  p.e = (float)(1.0 / sigma);
   if (kind == TSX_LJ) {
     p.a = (float)(sigma * sigma);
     p.b = (float)(4.0 * eps);
@@ -38,11 +39,16 @@ __host__ __device__ inline PairP make_pair_params(int kind, double sigma, double
 
 // Energy e and scalar fs with force vector f_ij = fs * dr_ij = -(dV/dr / r) dr_ij
 // (the force on j due to i, pair_potential.py:426-428).
+// Hardware reciprocal / square root (1 ulp): the pair arithmetic is FP32 by
+// design, and r^2 > 0 is never denormal here.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 template <int KIND>
 __device__ __forceinline__ void pair_eval(float r2, const PairP& p, float& e, float& fs) {
   if (KIND == TSX_LJ) {
     if (r2 > 0.f) {
-      const float ir2 = 1.0f / r2;
+      const float ir2 = frcp(r2);
       const float sr2 = p.a * ir2;
       const float sr6 = sr2 * sr2 * sr2;
       e = p.b * (sr6 * sr6 - sr6);
@@ -53,26 +59,27 @@ __device__ __forceinline__ void pair_eval(float r2, const PairP& p, float& e, fl
     }
   } else if (KIND == TSX_MORSE) {
     if (r2 > 0.f) {
-      const float r = __fsqrt_rn(r2);
+      const float r = fsqrt(r2);
       const float x = __expf(-p.c * (r - p.a));
       const float omx = 1.f - x;
       e = p.b * omx * omx - p.b;
-      fs = -p.d * x * omx / r;
+      fs = -p.d * x * omx * frcp(r);
     } else {
       e = 0.f;
       fs = 0.f;
     }
   } else {
-    const float r = __fsqrt_rn(r2);
+    const float r = fsqrt(r2);
     if (r < p.a) {
-      const float t = 1.f - r / p.a;
+      const float t = 1.f - r * p.e;
+      const float ir = r > 0.f ? frcp(r) : 0.f;
       if (p.c == 2.f) {
         e = 0.5f * p.b * t * t;
-        fs = r > 0.f ? p.d * t / r : 0.f;
+        fs = p.d * t * ir;
       } else {
         const float tm1 = powf(t, p.c - 1.f);
         e = p.b / p.c * tm1 * t;
-        fs = r > 0.f ? p.d * tm1 / r : 0.f;
+        fs = p.d * tm1 * ir;
       }
     } else {
       e = 0.f;
@@ -129,6 +136,10 @@ struct RowAcc {
 // mask is d^2 < rc^2 (pair_potential.py:194), the pair function FP32.
 // Species of atom j come from ``species`` (stride 1) or, when ``species`` is
 // NULL and ns > 1, from the packed fourth coordinate of a stride-4 position.
+// Entries are taken UNR at a time: all list words, then all positions are
+// requested before any is used, so each lane keeps UNR gathers in flight.
+constexpr int UNR = 4;
+
 template <int KIND, int TPA, bool VIRIAL, int PS>
 __device__ __forceinline__ RowAcc accumulate_row(
     const PairP* sp, int ns, const int32_t* __restrict__ species,
@@ -145,42 +156,59 @@ __device__ __forceinline__ RowAcc accumulate_row(
   const int bias = 1 << (sbits - 1);
   const uint32_t fmask = (1u << sbits) - 1u;
   const uint32_t jmask = (1u << jbits) - 1u;
-  for (int n = lane; n < cnt; n += TPA) {
-    const uint32_t w = nbr[ell_idx(t, n, TPA, ld)];
-    const int j = base + (int)(w & jmask);
-    const uint32_t sf = w >> jbits;
-    const int sx = (int)(sf & fmask) - bias;
-    const int sy = (int)((sf >> sbits) & fmask) - bias;
-    const int sz = (int)((sf >> (2 * sbits)) & fmask) - bias;
-    const double* pj = pos + PS * (int64_t)j;
-    double dx = pj[0] - xi;
-    double dy = pj[1] - yi;
-    double dz = pj[2] - zi;
-    if ((sx | sy | sz) != 0) {
-      const double fxs = (double)sx, fys = (double)sy, fzs = (double)sz;
-      dx += fxs * L00 + fys * L10 + fzs * L20;
-      dy += fxs * L01 + fys * L11 + fzs * L21;
-      dz += fxs * L02 + fys * L12 + fzs * L22;
+  // entry n = lane + TPA * k of this row lives at wp[k * stride]
+  const uint32_t* wp = nbr + t * TPA + lane;
+  const int64_t stride = ld * TPA;
+  const int iters = cnt > lane ? (cnt - lane + TPA - 1) / TPA : 0;
+  for (int k0 = 0; k0 < iters; k0 += UNR) {
+    uint32_t w[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) w[u] = (k0 + u < iters) ? wp[(int64_t)(k0 + u) * stride] : 0u;
+    double px[UNR], py[UNR], pz[UNR];
+    int tj[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const double* pj = pos + PS * (int64_t)(base + (int)(w[u] & jmask));
+      px[u] = pj[0];
+      py[u] = pj[1];
+      pz[u] = pj[2];
+      tj[u] = 0;
+      if (ns > 1) tj[u] = species ? species[base + (int)(w[u] & jmask)] : (int)__double_as_longlong(pj[3]);
     }
-    const double d2 = dx * dx + dy * dy + dz * dz;
-    if (!(d2 < rc2)) continue;
-    int tj = 0;
-    if (ns > 1) tj = species ? species[j] : (int)__double_as_longlong(pj[3]);
-    float e, fs;
-    pair_eval<KIND>((float)d2, sp[ti * ns + tj], e, fs);
-    const double f = (double)fs;
-    const double fvx = f * dx, fvy = f * dy, fvz = f * dz;
-    a.fx -= fvx;
-    a.fy -= fvy;
-    a.fz -= fvz;
-    a.e += 0.5 * (double)e;
-    if (VIRIAL) {
-      a.w[0] += 0.5 * dx * fvx;
-      a.w[1] += 0.5 * dy * fvy;
-      a.w[2] += 0.5 * dz * fvz;
-      a.w[3] += 0.5 * dy * fvz;
-      a.w[4] += 0.5 * dx * fvz;
-      a.w[5] += 0.5 * dx * fvy;
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      if (k0 + u >= iters) break;
+      const uint32_t sf = w[u] >> jbits;
+      double dx = px[u] - xi;
+      double dy = py[u] - yi;
+      double dz = pz[u] - zi;
+      if (sf != (((uint32_t)bias) | ((uint32_t)bias << sbits) |
+                 ((uint32_t)bias << (2 * sbits)))) {  // S' != 0
+        const double fxs = (double)((int)(sf & fmask) - bias);
+        const double fys = (double)((int)((sf >> sbits) & fmask) - bias);
+        const double fzs = (double)((int)((sf >> (2 * sbits)) & fmask) - bias);
+        dx += fxs * L00 + fys * L10 + fzs * L20;
+        dy += fxs * L01 + fys * L11 + fzs * L21;
+        dz += fxs * L02 + fys * L12 + fzs * L22;
+      }
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      if (!(d2 < rc2)) continue;
+      float e, fs;
+      pair_eval<KIND>((float)d2, sp[ti * ns + tj[u]], e, fs);
+      const double f = (double)fs;
+      a.fx = fma(-f, dx, a.fx);
+      a.fy = fma(-f, dy, a.fy);
+      a.fz = fma(-f, dz, a.fz);
+      a.e = fma(0.5, (double)e, a.e);
+      if (VIRIAL) {
+        const double hf = 0.5 * f;
+        a.w[0] += hf * dx * dx;
+        a.w[1] += hf * dy * dy;
+        a.w[2] += hf * dz * dz;
+        a.w[3] += hf * dy * dz;
+        a.w[4] += hf * dx * dz;
+        a.w[5] += hf * dx * dy;
+      }
     }
   }
 #pragma unroll
@@ -193,6 +221,16 @@ __device__ __forceinline__ RowAcc accumulate_row(
       for (int c = 0; c < 6; ++c) a.w[c] += __shfl_xor(a.w[c], off, 64);
   }
   return a;
+}
+
+// XCD-aware block order: consecutive hardware blocks are dealt round-robin
+// to the 8 XCDs of the chip (each with its own L2).  Renumbering so that
+// every XCD processes one contiguous range of rows keeps the neighbour
+// positions a block gathers in that XCD's L2.
+__device__ __forceinline__ int xcd_block() {
+  const int G = gridDim.x, b = blockIdx.x;
+  const int q = G >> 3, r = G & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
 }
 
 }  // namespace tsx

@@ -27,7 +27,7 @@ __global__ __launch_bounds__(256) void k_pair(
     sp[k] = make_pair_params(KIND, params[3 * k], params[3 * k + 1], params[3 * k + 2]);
   __syncthreads();
 
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gid = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   const int64_t t = gid / TPA;
   const int lane = (int)(gid % TPA);
   const bool valid = t < n_atoms;
