@@ -53,6 +53,7 @@ struct NLWs {
   int32_t* slot;       // N
   double* wr;          // N x 4 (wrapped xyz, packed wrap shift)
   int32_t* scan_tmp;   // scan block sums
+  int32_t* act;        // [S + 1] exclusive prefix of the bins of gated-on systems
 };
 
 static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -74,6 +75,7 @@ static size_t ws_layout(int64_t n, int32_t s, int32_t bin_cap, NLWs* w, char* ba
   void* p_sl = take(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
   void* p_wr = take(sizeof(double) * 4 * (size_t)(n > 0 ? n : 1));
   void* p_st = take(sizeof(int32_t) * 4096);
+  void* p_act = take(sizeof(int32_t) * ((size_t)(s > 0 ? s : 1) + 1));
   if (w) {
     w->geom = (double*)p_geom;
     w->grid = (int32_t*)p_grid;
@@ -84,6 +86,7 @@ static size_t ws_layout(int64_t n, int32_t s, int32_t bin_cap, NLWs* w, char* ba
     w->slot = (int32_t*)p_sl;
     w->wr = (double*)p_wr;
     w->scan_tmp = (int32_t*)p_st;
+    w->act = (int32_t*)p_act;
   }
   return off;
 }
@@ -327,6 +330,56 @@ __device__ __forceinline__ bool gated_off(const int32_t* __restrict__ gate, int 
 // (s, part) handles system s, striding its atoms / bins by B, so systems whose
 // gate is off cost one load per block.
 
+// ----------------------------------------------------------- active bins
+// One workgroup: act[s] = number of bins of the gated-on systems before s,
+// act[n_sys] = their total.  The sort and fill kernels are persistent grids
+// that walk this compact list of active bins.
+__global__ __launch_bounds__(1024) void k_active(const int32_t* __restrict__ grid,
+                                                 const int32_t* __restrict__ meta, int bin_cap,
+                                                 int n_sys, const int32_t* __restrict__ gate,
+                                                 int32_t* __restrict__ act) {
+  __shared__ int32_t sh[1024];
+  __shared__ int32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  const bool ok = meta[0] <= bin_cap;
+  for (int c0 = 0; c0 < n_sys; c0 += 1024) {
+    const int s = c0 + threadIdx.x;
+    int v = 0;
+    if (ok && s < n_sys && !gated_off(gate, s)) {
+      const int32_t* gr = grid + (size_t)GRID * s;
+      v = gr[0] * gr[1] * gr[2];
+    }
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+      const int t = threadIdx.x >= (unsigned)d ? sh[threadIdx.x - d] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (s < n_sys) act[s] = carry + sh[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += sh[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) act[n_sys] = carry;
+}
+
+// system owning active work item w: last s with act[s] <= w (act is
+// non-decreasing; systems with no active bins are skipped automatically)
+__device__ __forceinline__ int active_system(const int32_t* __restrict__ act, int n_sys, int w) {
+  int lo = 0, hi = n_sys - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (act[mid] <= w)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
 // -------------------------------------------------------------------- bin
 // Bin by wrapped fractional coordinate, atomic slot, and (engine) the
 // displacement reference copy of the position.
@@ -414,11 +467,10 @@ __global__ __launch_bounds__(256) void k_scatter(
 }
 
 // ---------------------------------------------------------- sort + wrap
-// Four independent waves per block, one bin at a time per wave (wave w of
-// block (s, part) takes bin off + part * 4 + w, then strides by 4 B): order
-// the bin's atoms by index (rank sort in the wave's LDS slice; atom indices
-// are unique), reset the bin counter for the next build, and write the
-// wrapped FP64 coordinates in bin order.
+// Persistent grid of four independent waves per block walking the active-bin
+// list: order each bin's atoms by index (rank sort in the wave's LDS slice;
+// atom indices are unique), reset the bin counter for the next build, and
+// write the wrapped FP64 coordinates in bin order.
 constexpr int SORT_W = 4;
 constexpr int SORT_CAP = 1024;
 
@@ -429,19 +481,20 @@ __device__ __forceinline__ void wave_sync_sort() {
 }
 
 __global__ __launch_bounds__(64 * SORT_W) void k_sort_wrap(
-    const double* __restrict__ pos, int ps, int n_sys, int B, const double* __restrict__ geom,
+    const double* __restrict__ pos, int ps, int n_sys, const double* __restrict__ geom,
     const int32_t* __restrict__ grid, const int32_t* __restrict__ meta, int bin_cap,
-    const int32_t* __restrict__ bin_start, int32_t* __restrict__ bin_count,
-    int32_t* __restrict__ order, double* __restrict__ wr, const int32_t* __restrict__ gate) {
+    const int32_t* __restrict__ act, const int32_t* __restrict__ bin_start,
+    int32_t* __restrict__ bin_count, int32_t* __restrict__ order, double* __restrict__ wr) {
   __shared__ int32_t keys_all[SORT_W][SORT_CAP];
-  const int s = blockIdx.x / B, part = blockIdx.x % B;
-  if (s >= n_sys || meta[0] > bin_cap || gated_off(gate, s)) return;
+  if (meta[0] > bin_cap) return;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int32_t* keys = keys_all[wv];
-  const int32_t* gr = grid + (size_t)GRID * s;
-  const double* g = geom + (size_t)GEOM * s;
-  const int off = gr[6], nbins = gr[0] * gr[1] * gr[2];
-  for (int b = off + part * SORT_W + wv; b < off + nbins; b += B * SORT_W) {
+  const int total = act[n_sys];
+  for (int w = blockIdx.x * SORT_W + wv; w < total; w += gridDim.x * SORT_W) {
+    const int s = active_system(act, n_sys, w);
+    const int32_t* gr = grid + (size_t)GRID * s;
+    const double* g = geom + (size_t)GEOM * s;
+    const int b = gr[6] + (w - act[s]);
     const int lo = bin_start[b], hi = bin_start[b + 1], n = hi - lo;
     if (n > 1 && n <= SORT_CAP) {
       for (int k = lane; k < n; k += 64) keys[k] = order[lo + k];
@@ -512,8 +565,8 @@ __device__ __forceinline__ int64_t ell_index(int64_t t, int n, int tpa, int64_t 
 
 
 // ------------------------------------------------------------------- fill
-// Four independent waves per block; each wave owns a stream of bins (wave w
-// of block (s, part) takes bin off + part * 4 + w, then strides by 4 B).
+// Persistent grid of four independent waves per block walking the active-bin
+// list (a wave takes one bin at a time).
 // All LDS traffic of a wave stays in its own LDS slice, so waves synchronise
 // with wave barriers only.  For one bin, the stencil images (bin + lattice
 // shift S, m_k bins each way) are enumerated in LDS and their atoms staged in
@@ -543,12 +596,12 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 }
 
 __global__ __launch_bounds__(64 * FILL_W) void k_fill(
-    const int32_t* __restrict__ sys_start, int n_sys, int B, const double* __restrict__ geom,
+    const int32_t* __restrict__ sys_start, int n_sys, const double* __restrict__ geom,
     const int32_t* __restrict__ grid, const int32_t* __restrict__ meta, int bin_cap,
-    const int32_t* __restrict__ bin_start, const int32_t* __restrict__ order,
-    const double* __restrict__ wr, double rl2, int self_int, int half, int jbits, int tpa,
-    int kmax, int64_t ld, uint32_t* __restrict__ nbr, int32_t* __restrict__ count,
-    int32_t* __restrict__ status, const int32_t* __restrict__ gate) {
+    const int32_t* __restrict__ act, const int32_t* __restrict__ bin_start,
+    const int32_t* __restrict__ order, const double* __restrict__ wr, double rl2, int self_int,
+    int half, int jbits, int tpa, int kmax, int64_t ld, uint32_t* __restrict__ nbr,
+    int32_t* __restrict__ count, int32_t* __restrict__ status) {
   __shared__ double4 cpos_all[FILL_W][FILL_CH];
   __shared__ int32_t cj_all[FILL_W][FILL_CH];
   __shared__ int32_t cimg_all[FILL_W][FILL_CH];
@@ -556,8 +609,7 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
   __shared__ int32_t is_all[FILL_W][FILL_IMG][3];
   __shared__ int32_t ibeg_all[FILL_W][FILL_IMG];
   __shared__ int32_t ipre_all[FILL_W][FILL_IMG + 1];
-  const int s = blockIdx.x / B, part = blockIdx.x % B;
-  if (s >= n_sys || meta[0] > bin_cap || gated_off(gate, s)) return;
+  if (meta[0] > bin_cap) return;
   const int wv = threadIdx.x >> 6, tid = threadIdx.x & 63;
   double4* cpos = cpos_all[wv];
   int32_t* cj = cj_all[wv];
@@ -569,13 +621,16 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
   const int sbits = (32 - jbits) / 3;
   const int bias = 1 << (sbits - 1);
   const uint32_t fmask = (1u << sbits) - 1u;
-  const double* g = geom + (size_t)GEOM * s;
-  const int32_t* gr = grid + (size_t)GRID * s;
-  const int base = sys_start[s];
-  const int off = gr[6], nbins = gr[0] * gr[1] * gr[2];
-  const int wx = 2 * gr[3] + 1, wy = 2 * gr[4] + 1, wz = 2 * gr[5] + 1;
-  const int n_img = wx * wy * wz;
-  for (int b = off + part * FILL_W + wv; b < off + nbins; b += B * FILL_W) {
+  const int total = act[n_sys];
+  for (int wi_ = blockIdx.x * FILL_W + wv; wi_ < total; wi_ += gridDim.x * FILL_W) {
+    const int s = active_system(act, n_sys, wi_);
+    const double* g = geom + (size_t)GEOM * s;
+    const int32_t* gr = grid + (size_t)GRID * s;
+    const int base = sys_start[s];
+    const int off = gr[6];
+    const int wx = 2 * gr[3] + 1, wy = 2 * gr[4] + 1, wz = 2 * gr[5] + 1;
+    const int n_img = wx * wy * wz;
+    const int b = off + (wi_ - act[s]);
     const int c0 = bin_start[b], c1 = bin_start[b + 1];
     const int nc = c1 - c0;
     if (nc == 0) continue;
@@ -805,7 +860,6 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
   // blocks per system: whole-grid parallelism for one large system, a few
   // blocks each for hundreds of small ones
   const int b_atoms = n_systems >= 2048 ? 1 : 2048 / n_systems;
-  const int b_bins = n_systems >= 8192 ? 1 : 8192 / n_systems;
   k_bin<<<n_systems * b_atoms, 256, 0, st>>>(pos, pos_stride, sys_start, n_systems, b_atoms,
                                              w.geom, w.grid, w.meta, bin_cap, w.bin_count,
                                              w.atom_bin, w.slot, gate, xref);
@@ -817,14 +871,19 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
                                                  bin_cap, w.atom_bin, w.slot, w.bin_start, order,
                                                  gate);
   TSX_CHECK_LAUNCH();
-  k_sort_wrap<<<n_systems * b_bins, 64 * SORT_W, 0, st>>>(pos, pos_stride, n_systems, b_bins, w.geom,
-                                                 w.grid, w.meta, bin_cap, w.bin_start,
-                                                 w.bin_count, order, w.wr, gate);
+  // persistent grids over the active-bin list (sizes fixed for graph capture)
+  k_active<<<1, 1024, 0, st>>>(w.grid, w.meta, bin_cap, n_systems, gate, w.act);
   TSX_CHECK_LAUNCH();
-  k_fill<<<n_systems * b_bins, 64 * FILL_W, 0, st>>>(sys_start, n_systems, b_bins, w.geom, w.grid,
-                                                w.meta, bin_cap, w.bin_start, order, w.wr,
-                                                r_list * r_list, self_interaction, half, jbits,
-                                                tpa, kmax, ld, nbr, count, status, gate);
+  const int g_sort = blocks_for(bin_cap, SORT_W) < 2048 ? blocks_for(bin_cap, SORT_W) : 2048;
+  k_sort_wrap<<<g_sort, 64 * SORT_W, 0, st>>>(pos, pos_stride, n_systems, w.geom, w.grid,
+                                              w.meta, bin_cap, w.act, w.bin_start, w.bin_count,
+                                              order, w.wr);
+  TSX_CHECK_LAUNCH();
+  const int g_fill = blocks_for(bin_cap, FILL_W) < 768 ? blocks_for(bin_cap, FILL_W) : 768;
+  k_fill<<<g_fill, 64 * FILL_W, 0, st>>>(sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap,
+                                         w.act, w.bin_start, order, w.wr, r_list * r_list,
+                                         self_interaction, half, jbits, tpa, kmax, ld, nbr,
+                                         count, status);
   TSX_CHECK_LAUNCH();
   return TSX_OK;
 }
