@@ -33,6 +33,16 @@ import torch_sim_amd as tsa  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 MORSE = dict(sigma=2.866, epsilon=0.3429, alpha=1.3588)
 
+# HBM bytes per dispatch of the fused-step kernel, from two rocprofv3 --pmc runs of this
+# script (FETCH_SIZE, WRITE_SIZE in KiB per dispatch, averaged over the dispatches), each
+# scaled by what the same counter reads for a 256 MiB device copy (FETCH_SIZE x 2.000,
+# WRITE_SIZE x 1.000).  Keyed by (n_systems, skin, tpa); the summary with its commands is
+# profiles/k_md_fused__cu_morse_nvt_512sys.md.
+MEASURED_TRAFFIC = {
+    (512, 0.72, 2): {"fetch_kib": 107098.6, "fetch_factor": 2.000, "write_kib": 36122.8,
+                     "write_factor": 1.000},
+}
+
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
@@ -184,6 +194,11 @@ def main() -> None:
     achieved = bytes_launch / (ms_kernel * 1e-3) / 1e9
     log(f"fused kernel: {ms_kernel * 1e3:.1f} us, k_list {k_mean:.1f}, {achieved:.0f} GB/s")
 
+    traffic = None
+    meas = MEASURED_TRAFFIC.get((args.n_systems, round(skin, 3), tpa))
+    if meas is not None:
+        traffic = 1024.0 * (meas["fetch_kib"] * meas["fetch_factor"] + meas["write_kib"] * meas["write_factor"])
+
     plain = None
     if not args.no_plain:
         plain = plain_gpu(state, model, args.plain_steps, 1)
@@ -233,7 +248,7 @@ def main() -> None:
             "frac": achieved / HBM_PEAK_GBS,
             "bytes_per_launch": bytes_launch,
             "kernel_ms": ms_kernel,
-            "traffic": None,
+            "traffic": traffic,
         },
         "cpu_baseline": cpu,
     }
