@@ -229,6 +229,60 @@ int tsx_md_reduce(const double* e_atom, const int32_t* sys_start, int32_t n_syst
 int tsx_copy_ref(const double* pos, double* xref, int64_t n_doubles, const int32_t* gate,
                  int32_t* counter, void* stream);
 
+
+/* ------------------------------------------------------------ FIRE ------ */
+
+#define TSX_FIRE_ASE 0      /* fire_flavor="ase_fire" (fire.py:288-499)    */
+#define TSX_FIRE_VV 1       /* fire_flavor="vv_fire"  (fire.py:179-285)    */
+#define TSX_CELL_NONE 0     /* positions only                              */
+#define TSX_CELL_UNIT 1     /* CellFilter.unit    (cell_filters.py:180-227) */
+#define TSX_CELL_FRECHET 2  /* CellFilter.frechet (cell_filters.py:230-284) */
+#define TSX_MAX_LOG_DEFORM 2.0 /* cell_filters.py:23 */
+
+/* fire_step keyword arguments (fire.py:117-128) plus the cell-filter options
+ * (cell_filters.py:230-240).  eps is 1e-16 for float64 (fire.py:149).
+ * first_step: every velocity is NaN (fire.py:313-315): raw forces, no mixing
+ * and no dt / alpha update on this step (ASE flavour). */
+typedef struct tsx_fire_params {
+  double dt_max, f_inc, f_dec, alpha_start, f_alpha, max_step, eps;
+  int32_t n_min, flavor, cell_filter, hydrostatic_strain, constant_volume, first_step;
+} tsx_fire_params;
+
+/* First half of a FIRE step, one workgroup per system (replaces the part of
+ * _ase_fire_step before the model call, fire.py:306-481, or of _vv_fire_step,
+ * fire.py:193-212).  Updates vel, pos (stride 3 or 4; column 3 untouched),
+ * dt, alpha, n_pos and, under a cell filter, cell_col, cell_pos, cell_vel.
+ * active (may be NULL): systems with active[s] == 0 are left untouched.
+ * List check (flags non-NULL): flags[s] = 1 when an atom of s, pulled back to
+ * the cell of the last list build (cell_build), moved more than the distance
+ * that keeps a list of radius r_list valid for cutoff.  ws: tsx_fire_ws_bytes
+ * of device scratch (per-system step data between the pre kernels). */
+int64_t tsx_fire_ws_bytes(int32_t n_systems);
+int tsx_fire_pre(const tsx_fire_params* prm, int32_t n_systems, const int32_t* sys_start,
+                 const int32_t* atom_sys, int64_t n_atoms, double* pos, int32_t pos_stride,
+                 double* vel, const double* forces, const double* masses, double* dt,
+                 double* alpha, int32_t* n_pos, double* cell_col, const double* ref_cell,
+                 const double* cell_factor, double* cell_pos, double* cell_vel,
+                 const double* cell_forces, const int32_t* active, const double* cell_build,
+                 const double* xref, double r_list, double cutoff, int32_t* flags, void* ws,
+                 void* stream);
+
+/* Second half of a FIRE step after the model call: cell forces from stress
+ * (compute_cell_forces, cell_filters.py:353-399; with init != 0 the filter
+ * init functions, cell_filters.py:180-284), the VV flavour's second half
+ * kick / power / mixing (fire.py:227-285, skipped when init != 0), and
+ * fmax[2*s] = max_i |F_i|, fmax[2*s+1] = max row norm of the cell forces
+ * (quantities.py:327-342, runners.py:527-534).  cell_factor [S], pressure and
+ * stress [S,3,3], cell_masses [S] (total mass).  flags (may be NULL): for
+ * systems whose list was rebuilt, cell_build[s] = cell[s] and flags[s] = 0. */
+int tsx_fire_post(const tsx_fire_params* prm, int32_t n_systems, const int32_t* sys_start,
+                  double* vel, const double* forces, const double* masses, double* dt,
+                  double* alpha, int32_t* n_pos, const double* cell_col, const double* ref_cell,
+                  const double* cell_factor, const double* pressure, const double* stress,
+                  double* cell_vel, double* cell_forces, const double* cell_masses,
+                  const int32_t* active, double* fmax, double* cell_build, int32_t* flags,
+                  int32_t init, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,9 +34,13 @@ def fire_init(
     dt_start=0.1,
     alpha_start=0.1,
     frechet=False,
+    unit=False,
     scalar_pressure=0.0,
+    hydrostatic_strain=False,
+    constant_volume=False,
 ):
-    """fire_init + frechet_cell_filter_init (fire.py:21-114, cell_filters.py:230-284)."""
+    """fire_init + frechet / unit cell-filter init (fire.py:21-114,
+    cell_filters.py:180-284)."""
     n_sys = cell_col.shape[0]
     out = model_fn(positions, cell_col)
     st = {
@@ -52,31 +56,52 @@ def fire_init(
         "dt": torch.full((n_sys,), float(dt_start), dtype=positions.dtype),
         "alpha": torch.full((n_sys,), float(alpha_start), dtype=positions.dtype),
         "n_pos": torch.zeros(n_sys, dtype=torch.int32),
-        "frechet": frechet,
+        "frechet": frechet or unit,
+        "filter": "frechet" if frechet else ("unit" if unit else None),
+        "hydrostatic_strain": hydrostatic_strain,
+        "constant_volume": constant_volume,
     }
-    if frechet:
+    if frechet or unit:
         counts = torch.bincount(system_idx, minlength=n_sys).to(positions.dtype)
         st["cell_factor"] = counts.view(n_sys, 1, 1)
         st["pressure"] = (scalar_pressure * torch.eye(3, dtype=positions.dtype)).expand(
             n_sys, 3, 3
         )
         st["reference_cell"] = cell_col.clone()
-        st["cell_positions"] = torch.zeros((n_sys, 3, 3), dtype=positions.dtype)
         vol = torch.linalg.det(cell_col).view(n_sys, 1, 1)
-        virial = -vol * (out["stress"] + st["pressure"])
+        virial = _apply_constraints(-vol * (out["stress"] + st["pressure"]), st)
         f = deform_grad(cell_col.mT, cell_col.mT)
-        ucf = torch.bmm(virial, torch.linalg.inv(f.transpose(1, 2)))
-        st["cell_forces"] = ucf / st["cell_factor"]
+        if frechet:
+            st["cell_positions"] = torch.zeros((n_sys, 3, 3), dtype=positions.dtype)
+            ucf = torch.bmm(virial, torch.linalg.inv(f.transpose(1, 2)))
+            st["cell_forces"] = ucf / st["cell_factor"]
+        else:  # cell_filters.py:180-227
+            st["cell_positions"] = f * st["cell_factor"]
+            st["cell_forces"] = virial / st["cell_factor"]
         st["cell_velocities"] = torch.full_like(st["cell_forces"], float("nan"))
     return st
 
 
+def _apply_constraints(virial, st):
+    """Hydrostatic-strain / constant-volume virial projections (cell_filters.py:117-134)."""
+    eye = torch.eye(3, dtype=virial.dtype)
+    if st.get("hydrostatic_strain"):
+        virial = torch.diagonal(virial, dim1=1, dim2=2).mean(dim=1).view(-1, 1, 1) * eye
+    if st.get("constant_volume"):
+        virial = virial - torch.diagonal(virial, dim1=1, dim2=2).mean(dim=1).view(-1, 1, 1) * eye
+    return virial
+
+
 def compute_cell_forces(st, out):
-    """Frechet branch of compute_cell_forces (cell_filters.py:353-399)."""
+    """compute_cell_forces, Frechet and unit branches (cell_filters.py:353-399)."""
     n = st["n_sys"]
     vol = torch.linalg.det(st["cell"]).view(n, 1, 1)
-    virial = -vol * (out["stress"] + st["pressure"])
+    virial = _apply_constraints(-vol * (out["stress"] + st["pressure"]), st)
     f = deform_grad(st["reference_cell"].mT, st["cell"].mT)
+    if st["filter"] == "unit":
+        vt = torch.linalg.solve(f, virial.transpose(-2, -1)).transpose(-2, -1)
+        st["cell_forces"] = vt / st["cell_factor"]
+        return
     ucf = torch.bmm(virial, torch.linalg.inv(f.transpose(1, 2)))
     log_f = mat33.logm(f).clamp(-MAX_LOG_DEFORM, MAX_LOG_DEFORM)
     st["cell_forces"] = mat33.frechet_cell_forces(log_f, ucf) / st["cell_factor"]
@@ -165,14 +190,20 @@ def ase_fire_step(
         frac = torch.linalg.solve(cur[idx], st["positions"].unsqueeze(-1)).squeeze(-1)
         new_frac = frac + dr
         cp = st["cell_positions"] + drc
-        log_new = cp / st["cell_factor"]
-        if bool((log_new.abs() > MAX_LOG_DEFORM).any()):
-            log_new = log_new.clamp(-MAX_LOG_DEFORM, MAX_LOG_DEFORM)
-            cp = log_new * st["cell_factor"]
-        f_new = mat33.expm(log_new)
+        if st["filter"] == "unit":
+            f_new = cp / st["cell_factor"]
+        else:
+            log_new = cp / st["cell_factor"]
+            if bool((log_new.abs() > MAX_LOG_DEFORM).any()):
+                log_new = log_new.clamp(-MAX_LOG_DEFORM, MAX_LOG_DEFORM)
+                cp = log_new * st["cell_factor"]
+            f_new = mat33.expm(log_new)
         st["cell"] = torch.bmm(f_new, ref)
         adj = deform_grad(ref.mT, st["cell"].mT)
-        st["cell_positions"] = mat33.logm(adj) * st["cell_factor"]
+        if st["filter"] == "unit":
+            st["cell_positions"] = adj * st["cell_factor"]
+        else:
+            st["cell_positions"] = mat33.logm(adj) * st["cell_factor"]
         st["positions"] = torch.bmm(new_frac.unsqueeze(1), adj[idx].transpose(-2, -1)).squeeze(1)
     else:
         st["positions"] = st["positions"] + dr

@@ -35,3 +35,19 @@ def test_host_side_queries():
     assert lib.tsx_nl_ws_bytes(1000, 4, 100) > 1000 * 4 * 8
     assert lib.tsx_scan_ws_bytes(10**6) >= 8 * 2 * (10**6 // 2048)
     assert lib.tsx_last_error() is not None
+
+
+def _param_counts():
+    text = open(HEADER).read()
+    out = {}
+    for m in re.finditer(r"^(?:int|int64_t|const char\*)\s+(tsx_\w+)\(([^;]*)\);", text, re.M | re.S):
+        args = m.group(2).strip()
+        out[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+def test_binding_arity_matches_header():
+    counts = _param_counts()
+    bad = {n: (c, len(tsa._lib._SIGS[n][1])) for n, c in counts.items()
+           if len(tsa._lib._SIGS[n][1]) != c}
+    assert not bad, bad

@@ -32,8 +32,23 @@ from .models import (
 )
 from .neighbors import torchsim_nl
 from .quantities import calc_kinetic_energy, calc_kT, system_wise_max_force
-from .runners import integrate
-from .state import SimState, concatenate_states, initialize_state
+from .optimizers import (
+    OPTIM_REGISTRY,
+    CellFilter,
+    CellFireState,
+    FireState,
+    Optimizer,
+    fire_init,
+    fire_step,
+)
+from .runners import (
+    InFlightAutoBatcher,
+    generate_energy_convergence_fn,
+    generate_force_convergence_fn,
+    integrate,
+    optimize,
+)
+from .state import SimState, concatenate_states, initialize_state, select_systems
 from .units import MetalUnits
 
 __version__ = "0.1.0"

@@ -58,7 +58,36 @@ _SIGS = {
     ),
     "tsx_md_reduce": (I, [P, P, I, P, P, I, P, P]),
     "tsx_copy_ref": (I, [P, P, L, P, P, P]),
+    "tsx_fire_ws_bytes": (L, [I]),
+    "tsx_fire_pre": (
+        I,
+        [P, I, P, P, L, P, I, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, D, D, P, P, P],
+    ),
+    "tsx_fire_post": (
+        I,
+        [P, I, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, I, P],
+    ),
 }
+
+
+class FireParams(ctypes.Structure):
+    """tsx_fire_params (include/tsx.h)."""
+
+    _fields_ = [
+        ("dt_max", ctypes.c_double),
+        ("f_inc", ctypes.c_double),
+        ("f_dec", ctypes.c_double),
+        ("alpha_start", ctypes.c_double),
+        ("f_alpha", ctypes.c_double),
+        ("max_step", ctypes.c_double),
+        ("eps", ctypes.c_double),
+        ("n_min", ctypes.c_int32),
+        ("flavor", ctypes.c_int32),
+        ("cell_filter", ctypes.c_int32),
+        ("hydrostatic_strain", ctypes.c_int32),
+        ("constant_volume", ctypes.c_int32),
+        ("first_step", ctypes.c_int32),
+    ]
 
 _lib = None
 
