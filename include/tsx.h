@@ -283,6 +283,42 @@ int tsx_fire_post(const tsx_fire_params* prm, int32_t n_systems, const int32_t* 
                   const int32_t* active, double* fmax, double* cell_build, int32_t* flags,
                   int32_t init, void* stream);
 
+
+/* ------------------------------------------------- backward (gradients) --- */
+
+/* Workspace bytes for tsx_pair_grad (block partials of the parameter
+ * gradient and per-atom cell gradients). */
+int64_t tsx_grad_ws_bytes(int64_t n_atoms, int32_t tpa, int32_t n_species);
+
+/* Gradients of the pair model over the full ELL list of tsx_nl_build
+ * (replaces torch.autograd through PairPotentialModel.forward with
+ * create_graph=True, torch_sim/models/pair_potential.py:351-464).
+ * With w [N,3] (cotangent of the forces): the vector-Jacobian product of the
+ * forces, dpos[i] = sum_j H(d_ij) (w_j - w_i) (a Hessian-vector product),
+ * dcell = d<w,F>/d cell, dparams = d<w,F>/d params.  With w == NULL: the
+ * gradient of sum_s ebar[s] E_s (ebar NULL = all ones): dpos = -ebar F,
+ * dcell, dparams.  params / dparams are [n_species, n_species, 3]
+ * (sigma, epsilon, alpha); dparams is returned symmetrised, so
+ * dparams[a,b] + dparams[b,a] (a != b) is the derivative with respect to
+ * the shared parameter of the species pair.  dcell is the gradient with
+ * respect to the column-vector cell [S,3,3] at fixed Cartesian positions.
+ * Any of dpos / dcell / dparams may be NULL; accumulate != 0 adds into dcell
+ * and dparams instead of overwriting (dpos is always overwritten).  Parameter gradients support up to 4
+ * species; sums are fixed-order (bitwise reproducible). */
+int tsx_pair_grad(int32_t kind, const double* params, int32_t n_species, const int32_t* species,
+                  const double* pos, int32_t pos_stride, const double* cell_col,
+                  const int32_t* sys_start, const int32_t* atom_sys, int32_t n_systems,
+                  const uint32_t* nbr, const int32_t* count, const int32_t* order,
+                  int64_t n_atoms, int32_t jbits, int32_t tpa, int64_t ld, double cutoff,
+                  const double* w, const double* ebar, double* dpos, double* dcell,
+                  double* dparams, int32_t accumulate, void* ws, void* stream);
+
+/* One reverse step of the discrete adjoint of velocity Verlet in momentum
+ * form (torch_sim/integrators/nve.py:74-130): xbar += vjp;
+ * pbar += (dt / m) xbar; w = wscale * pbar (the next force-VJP cotangent). */
+int tsx_nve_adjoint_update(double* xbar, double* pbar, const double* vjp, const double* masses,
+                           double dt, double wscale, double* w, int64_t n_atoms, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,6 +63,12 @@ _SIGS = {
         I,
         [P, I, P, P, L, P, I, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, D, D, P, P, P],
     ),
+    "tsx_grad_ws_bytes": (L, [L, I, I]),
+    "tsx_pair_grad": (
+        I,
+        [I, P, I, P, P, I, P, P, P, I, P, P, P, L, I, I, L, D, P, P, P, P, P, I, P, P],
+    ),
+    "tsx_nve_adjoint_update": (I, [P, P, P, P, D, D, P, L, P]),
     "tsx_fire_post": (
         I,
         [P, I, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, I, P],
