@@ -143,23 +143,8 @@ def cpu_baseline(n_sys=12, steps=2, seed=0):
     }
 
 
-def main() -> None:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--n-systems", type=int, default=512)
-    ap.add_argument("--skin", type=float, default=None)
-    ap.add_argument("--tpa", type=int, default=None)
-    ap.add_argument("--plain-steps", type=int, default=5)
-    ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-systems", type=int, default=12)
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU; runs on more than one GPU are not part of this build "
-              "(independent systems shard across GPUs without communication).", file=sys.stderr)
-        sys.exit(2)
+def bench_cu_morse(args) -> dict:
+    """512-system Cu Morse NVT Langevin (the headline workload)."""
     dev = torch.device("cuda:0")
     state = tsa.workloads.cu_batch(args.n_systems, device=dev, seed=0)
     model = tsa.MorseModel(**MORSE, cutoff=6.0, device=dev)
@@ -252,7 +237,235 @@ def main() -> None:
         },
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def _line(metric, value, unit, steps, warmup, ms, config, **extra) -> dict:
+    out = {
+        "metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32-pair/f64-accum", "data": "synthetic (seeded)",
+        "config": config,
+    }
+    out.update(extra)
+    return out
+
+
+def bench_lj1m(args) -> dict:
+    """1,048,576-atom LJ fluid, NVE, reduced units (rho* 0.8442, T* 0.72, rc 2.5,
+    skin 0.3, dt 0.005 tau)."""
+    dev = torch.device("cuda:0")
+    state = tsa.workloads.lj_fluid(64, device=dev, rho=0.8442, jitter=0.05, seed=0)
+    model = tsa.LennardJonesModel(sigma=1.0, epsilon=1.0, cutoff=2.5, device=dev)
+    state.rng = 0
+    md = tsa.nve_init(state=state, model=model, kT=0.72)
+    tpa = args.tpa or 4
+    eng = tsa.BatchedMD(model, md, dt=0.005, langevin=False, skin=args.skin or 0.3, tpa=tpa,
+                        seed=1, chunk_steps=250)
+    n = state.n_atoms
+    eng.run(args.warmup)
+    torch.cuda.synchronize()
+    reb0 = eng.n_rebuilds
+    t0 = time.perf_counter()
+    eng.run(args.steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    value = n * args.steps / el
+    ms_kernel = time_fused_kernel(eng)
+    bytes_launch = eng.algorithmic_bytes_per_atom_step() * n
+    achieved = bytes_launch / (ms_kernel * 1e-3) / 1e9
+    log(f"lj1m: {value:.4e} atom-steps/s, fused kernel {ms_kernel * 1e3:.1f} us, {achieved:.0f} GB/s")
+    return _line(
+        "MD atom-steps/s at 1 MI355X, 1M-atom LJ NVE", value, "atom-steps/s", args.steps,
+        args.warmup, 1e3 * el / args.steps,
+        {"workload": "lj_fluid_1m_nve", "n_atoms": n, "rho": 0.8442, "T": 0.72, "cutoff": 2.5,
+         "skin": args.skin or 0.3, "tpa": tpa, "dt_tau": 0.005,
+         "list_rebuilds_in_timed_steps": eng.n_rebuilds - reb0,
+         "mean_list_neighbors": round(eng.mean_neighbors, 2)},
+        roofline={"bound": "hbm", "kernel": "k_md_fused (tsx_md_fused_step)", "achieved": achieved,
+                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                  "bytes_per_launch": bytes_launch, "kernel_ms": ms_kernel, "traffic": None},
+    )
+
+
+def bench_ar256(args) -> dict:
+    """Ar fcc 4x4x4 (256 atoms) LJ NVE, 50 K, dt 2 fs, 100 steps; the FP64 CPU
+    oracle runs the same 100 steps as the baseline."""
+    from oracle import md as omd
+    from oracle import models as om
+    from oracle import nl as onl
+
+    dev = torch.device("cuda:0")
+    state = tsa.workloads.ar_fcc(device=dev)
+    model = tsa.LennardJonesModel(sigma=3.405, epsilon=0.0104, cutoff=8.5125, device=dev)
+    kT = 50.0 * tsa.MetalUnits.temperature
+    dt = 0.002 * tsa.MetalUnits.time
+    state.rng = 0
+    md = tsa.nve_init(state=state, model=model, kT=kT)
+    eng = tsa.BatchedMD(model, md, dt=dt, langevin=False, skin=1.0, tpa=8, seed=1,
+                        chunk_steps=100)
+    eng.run(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.run(args.steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    value = 256 * args.steps / el
+    cpu = None
+    if not args.no_cpu:
+        cst = tsa.workloads.ar_fcc(device="cpu")
+        fn = lambda d, zi, zj: om.lennard_jones_pair(d, zi, zj, 3.405, 0.0104)  # noqa: E731
+
+        def force(x):
+            o = om.pair_forward(x, cst.cell, cst.pbc, cst.atomic_numbers, cst.system_idx, fn,
+                                8.5125, compute_stress=False, nl=onl.linked_cell_nl)
+            return o["energy"], o["forces"]
+
+        torch.set_num_threads(os.cpu_count() or 1)
+        g = torch.Generator().manual_seed(0)
+        p = omd.initialize_momenta(cst.masses, cst.system_idx, torch.tensor(50.0 * omd.KB_EV),
+                                   torch.randn(256, 3, generator=g, dtype=torch.float64))
+        x = cst.positions.clone()
+        e, f = force(x)
+        x, p, f, e = omd.nve_step(x, p, f, cst.masses, force, 0.002 * omd.TIME_UNIT)  # warm-up
+        tc = time.perf_counter()
+        for _ in range(100):
+            x, p, f, e = omd.nve_step(x, p, f, cst.masses, force, 0.002 * omd.TIME_UNIT)
+        elc = time.perf_counter() - tc
+        cpu = {"value": 256 * 100 / elc, "unit": "atom-steps/s", "cores": torch.get_num_threads(),
+               "kind": "port", "sample": f"full 100 NVE steps, FP64 CPU oracle, {elc:.2f} s, "
+               f"cpu={platform.processor() or platform.machine()}"}
+    log(f"ar256: {value:.4e} atom-steps/s")
+    return _line("MD atom-steps/s at 1 MI355X, Ar-256 LJ NVE", value, "atom-steps/s", args.steps,
+                 args.warmup, 1e3 * el / args.steps,
+                 {"workload": "ar_fcc_256_nve", "n_atoms": 256, "T_K": 50.0, "dt_fs": 2.0,
+                  "cutoff_A": 8.5125}, cpu_baseline=cpu)
+
+
+def bench_fire(args) -> dict:
+    """1024 binary soft-sphere systems (64-1024 atoms, phi 0.64, triclinic
+    strain <= 5 %), ASE-FIRE with the Frechet cell filter: FIRE steps of the
+    whole batch, and how many systems reach fmax < 1e-4 (atoms and cell)."""
+    dev = torch.device("cuda:0")
+    state = tsa.workloads.soft_sphere_batch(args.n_systems or 1024, n_min=64, n_max=1024,
+                                            device=dev, seed=0)
+    sig = torch.tensor([[1.0, 1.2], [1.2, 1.4]], dtype=torch.float64)
+    skin = args.skin if args.skin is not None else 0.3
+    model = tsa.SoftSphereMultiModel(atomic_numbers=torch.tensor([1, 2]), sigma_matrix=sig,
+                                     device=dev, compute_stress=True, skin=skin)
+    fs = tsa.fire_init(state, model, cell_filter="frechet")
+    for _ in range(args.warmup):
+        tsa.fire_step(fs, model)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tsa.fire_step(fs, model)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    n = state.n_atoms
+    conv = int(((fs.fmax[:, 0] < 1e-4) & (fs.fmax[:, 1] < 1e-4)).sum())
+    value = n * args.steps / el
+    log(f"fire: {value:.4e} atom-steps/s, {conv} systems converged after {args.warmup + args.steps} steps")
+    return _line("FIRE atom-steps/s at 1 MI355X, 1024-system soft-sphere FIRE + Frechet", value,
+                 "atom-steps/s", args.steps, args.warmup, 1e3 * el / args.steps,
+                 {"workload": "soft_sphere_fire_frechet_1024", "n_systems": state.n_systems,
+                  "n_atoms": n, "phi": 0.64, "fmax": 1e-4, "skin": skin,
+                  "full_list_builds": model._vl.full_builds if model._vl else None,
+                  "converged_after_steps": conv, "steps_taken": args.warmup + args.steps})
+
+
+def bench_fit(args) -> dict:
+    """256 binary LJ systems (128-512 atoms, rho* 0.8, Kob-Andersen sigma /
+    epsilon): gradient of force-MSE (targets from parameters scaled by 1.03 /
+    0.95) + final potential energy after 200 NVE steps (dt 0.002 tau,
+    checkpoint every 20)."""
+    from torch_sim_amd import autograd as tad
+
+    dev = torch.device("cuda:0")
+    state = tsa.workloads.binary_lj_batch(args.n_systems or 256, n_min=128, n_max=512, rho=0.8,
+                                          device=dev, seed=0)
+    sig0 = torch.tensor([[1.0, 0.8], [0.8, 0.88]], dtype=torch.float64)
+    eps0 = torch.tensor([[1.0, 1.5], [1.5, 0.5]], dtype=torch.float64)
+    model = tsa.MultiLennardJonesModel(torch.tensor([1, 2]), sig0, eps0, device=dev, cutoff=2.5)
+    skin = args.skin if args.skin is not None else 0.3
+    terms = tad.PairTerms(model, skin=skin)
+    geo = tad.geometry(terms, state)
+    zero = torch.zeros(2, 2, dtype=torch.float64, device=dev)
+    with torch.no_grad():
+        target = tad.pair_forces(state.positions, state.cell,
+                                 tad.pair_params(sig0.to(dev) * 1.03, eps0.to(dev) * 0.95, zero), geo)
+    g = torch.Generator(device=dev).manual_seed(0)
+    p0 = torch.randn(state.positions.shape, generator=g, device=dev, dtype=torch.float64)
+    n = state.n_atoms
+    n_md = args.fit_md_steps
+
+    def grad_eval(with_md: bool):
+        sig = sig0.to(dev).clone().requires_grad_(True)
+        eps = eps0.to(dev).clone().requires_grad_(True)
+        params = tad.pair_params(sig, eps, zero)
+        f = tad.pair_forces(state.positions, state.cell, params, geo)
+        loss = ((f - target) ** 2).mean()
+        if with_md:
+            xn, _ = tad.nve_trajectory(state.positions, p0, params, geo, state.masses, state.cell,
+                                       dt=0.002, n_steps=n_md, checkpoint_every=20)
+            loss = loss + tad.pair_energy(xn, state.cell, params, geo).sum() / n
+        loss.backward()
+        return sig.grad, eps.grad
+
+    def timed(with_md, k, w):
+        for _ in range(w):
+            grad_eval(with_md)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            grad_eval(with_md)
+        torch.cuda.synchronize()
+        return k / (time.perf_counter() - t0)
+
+    fm = timed(False, 20, 2)
+    full = timed(True, args.steps, min(args.warmup, 1))
+    log(f"fit: force-matching {fm:.2f} grad-evals/s, full loss ({n_md} NVE steps) {full:.3f} grad-evals/s")
+    return _line("grad-evals/s at 1 MI355X, 256-system binary-LJ fit (force MSE + E after NVE)",
+                 full, "grad-evals/s", args.steps, min(args.warmup, 1), 1e3 / full,
+                 {"workload": "binary_lj_fit_256", "n_systems": state.n_systems, "n_atoms": n,
+                  "nve_steps": n_md, "checkpoint_every": 20, "dt_tau": 0.002, "skin": skin,
+                  "force_matching_only_grad_evals_per_s": fm})
+
+
+WORKLOADS = {
+    "cu_morse_nvt": (bench_cu_morse, 1000, 50),
+    "lj1m_nve": (bench_lj1m, 500, 20),
+    "ar256_nve": (bench_ar256, 100, 10),
+    "fire_ss_1024": (bench_fire, 100, 10),
+    "lj_fit_256": (bench_fit, 3, 1),
+}
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="cu_morse_nvt")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--n-systems", type=int, default=None)
+    ap.add_argument("--skin", type=float, default=None)
+    ap.add_argument("--tpa", type=int, default=None)
+    ap.add_argument("--plain-steps", type=int, default=5)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-systems", type=int, default=12)
+    ap.add_argument("--fit-md-steps", type=int, default=200)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU; runs on more than one GPU are not part of this build "
+              "(independent systems shard across GPUs without communication).", file=sys.stderr)
+        sys.exit(2)
+    fn, steps, warmup = WORKLOADS[args.workload]
+    args.steps = steps if args.steps is None else args.steps
+    args.warmup = warmup if args.warmup is None else args.warmup
+    if args.workload == "cu_morse_nvt" and args.n_systems is None:
+        args.n_systems = 512
+    print(json.dumps(fn(args)), flush=True)
 
 
 if __name__ == "__main__":

@@ -33,13 +33,14 @@
  *       soft_sphere.py:35-59 and 90-167
  *   tsx_md_* / tsx_md_fused_step
  *       torch_sim/integrators/md.py:113-208, nve.py:74-130, nvt.py:27-240
- *   tsx_fire_* / tsx_cell_*
- *       torch_sim/optimizers/fire.py:117-499,
- *       torch_sim/optimizers/cell_filters.py:137-178, 353-399,
- *       torch_sim/math.py:30-224, 614-778
- *   tsx_pair_backward_*
- *       no reference kernel: gradients of force-matching losses (the
- *       reference detaches dV/dr, pair_potential.py:419-425)
+ *   tsx_fire_pre / tsx_fire_post
+ *       torch_sim/optimizers/fire.py:21-499,
+ *       torch_sim/optimizers/cell_filters.py:26-60, 137-178, 180-284, 353-399,
+ *       torch_sim/math.py:106-189, 614-778
+ *   tsx_pair_grad / tsx_nve_adjoint_update
+ *       autograd through pair_potential.py:351-464 with create_graph=True
+ *       (the reference itself detaches dV/dr, pair_potential.py:419-425) and
+ *       through the NVE loop of nve.py:74-130
  */
 #ifndef TSX_H_
 #define TSX_H_
@@ -106,7 +107,8 @@ const char* tsx_last_error(void);
  * therefore carry a rebuild that runs only for the systems whose atoms left
  * the Verlet skin.  ``counter`` (may be NULL) is incremented once per rebuilt
  * system.  ``xref`` (stride 4, may be NULL) receives a copy of the positions
- * of the rebuilt systems (the displacement reference of the skin).
+ * of the rebuilt systems (the displacement reference of the skin) and
+ * ``cell_build`` ([S,3,3], may be NULL) a copy of their cells.
  */
 int64_t tsx_nl_ws_bytes(int64_t n_atoms, int32_t n_systems, int32_t bin_cap);
 
@@ -117,7 +119,7 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
                  int32_t half, int32_t jbits, int32_t tpa, int32_t kmax, int64_t ld,
                  int32_t bin_cap, int32_t setup, void* ws, uint32_t* nbr, int32_t* count,
                  int32_t* order, double* xref, int32_t* status, const int32_t* gate,
-                 int32_t* counter, void* stream);
+                 int32_t* counter, double* cell_build, void* stream);
 
 /* Exclusive scan of ``count`` into ``row_ptr`` [n+1] (int64); the total is
  * row_ptr[n].  Used before tsx_nl_emit_pairs.  ``ws`` holds
@@ -134,6 +136,19 @@ int tsx_nl_emit_pairs(const uint32_t* nbr, const int32_t* count, const int32_t* 
                       const int32_t* atom_sys, int64_t n_atoms, int32_t jbits,
                       int32_t tpa, int64_t ld, int64_t* mapping, int64_t n_pairs,
                       int64_t* system_mapping, double* shifts, void* stream);
+
+/* Verlet-skin validity of a list built at cell_build [S,3,3] (column) with
+ * positions xref [N,4] and radius r_list, for the current positions and
+ * cells: flags[s] = 1 when system s must be rebuilt (an atom pulled back to
+ * the build cell moved more than (r_list - cutoff / sigma) / 2, sigma a lower
+ * bound of the cell map's smallest singular value), else 0.  Replaces the
+ * rebuild-every-call of the reference NL (neighbors/__init__.py:46-98) for
+ * callers that keep a list with skin.  ws: tsx_nl_check_ws_bytes. */
+int64_t tsx_nl_check_ws_bytes(int32_t n_systems);
+int tsx_nl_check(const double* pos, int32_t pos_stride, const double* cell_col,
+                 const double* cell_build, const double* xref, const int32_t* atom_sys,
+                 int64_t n_atoms, int32_t n_systems, double r_list, double cutoff,
+                 int32_t* flags, void* ws, void* stream);
 
 /* Wrapped positions [N,3] and integer lattice shifts [N,3] (float64), as
  * pbc_wrap_batched_and_get_lattice_shifts (transforms.py:160-210).  ``ws``
@@ -170,11 +185,19 @@ int tsx_pair_forces(int32_t kind, const double* params, int32_t n_species,
 /* Per-system reductions of per-atom energies and virials:
  *   energy [S] = sum e_atom;  stress [S,3,3] = -(sum w_atom) / |det cell|;
  *   stresses [N,3,3] (optional) = -w_atom / |det cell|.
- * One workgroup per system, fixed summation order (deterministic). */
+ * Chunks of 2048 atoms counted from each system's first atom reduce in
+ * parallel (one 1M-atom system or many small ones); a system's sum order
+ * depends only on its size, so it is bitwise the same alone or in a batch. */
 int tsx_system_reduce(const double* e_atom, const double* w_atom, const double* cell_col,
-                      const int32_t* sys_start, const int32_t* atom_sys, int64_t n_atoms,
-                      int32_t n_systems, double* energy, double* stress,
-                      double* stresses, void* stream);
+                      const int32_t* sys_start, int32_t max_system_atoms, int32_t n_systems,
+                      double* energy, double* stress, double* stresses, void* ws,
+                      void* stream);
+
+/* Workspace bytes for tsx_system_reduce / tsx_md_reduce (max_system_atoms =
+ * the largest system).  The workspace must be zeroed once before its first
+ * use; every call leaves its counter at zero again (so calls can be captured
+ * in a graph and replayed). */
+int64_t tsx_reduce_ws_bytes(int32_t max_system_atoms, int32_t n_systems);
 
 
 /* ------------------------------------------------------------------ MD --
@@ -216,12 +239,13 @@ int tsx_md_fused_step(int32_t kind, const double* params, int32_t n_species,
                       uint64_t seed, const int64_t* step, int32_t end_kick, int32_t do_next,
                       double half_skin, int32_t* flag_set, void* stream);
 
-/* Per-system energy [S] = sum of e_atom (one workgroup per system, fixed
- * order); gate_clear[s] = 0 when given (the step's rebuild gates, read by the
- * list build before the fused step); when ``advance``, *step += 1. */
-int tsx_md_reduce(const double* e_atom, const int32_t* sys_start, int32_t n_systems,
-                  double* energy, int64_t* step, int32_t advance, int32_t* gate_clear,
-                  void* stream);
+/* Per-system energy [S] = sum of e_atom (the chunked fixed-order reduction
+ * of tsx_system_reduce, same workspace); gate_clear[s] = 0 when given (the
+ * step's rebuild gates, read by the list build before the fused step); when
+ * ``advance``, *step += 1. */
+int tsx_md_reduce(const double* e_atom, const int32_t* sys_start, int32_t max_system_atoms,
+                  int32_t n_systems, double* energy, int64_t* step, int32_t advance,
+                  int32_t* gate_clear, void* ws, void* stream);
 
 /* Copy n_doubles of pos into xref when *gate != 0 (or gate is NULL) and count
  * the copies in *counter (may be NULL): the displacement reference of the

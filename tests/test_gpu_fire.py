@@ -23,10 +23,11 @@ F64 = torch.float64
 SIG = torch.tensor([[1.0, 1.2], [1.2, 1.4]], dtype=F64)
 
 
-def _ss(dev, n_sys=3, n_min=32, n_max=64, seed=5):
+def _ss(dev, n_sys=3, n_min=32, n_max=64, seed=5, skin=0.0):
     st = tsa.workloads.soft_sphere_batch(n_sys, n_min=n_min, n_max=n_max, device=dev, seed=seed)
     model = tsa.SoftSphereMultiModel(
-        atomic_numbers=torch.tensor([1, 2]), sigma_matrix=SIG, device=dev, compute_stress=True
+        atomic_numbers=torch.tensor([1, 2]), sigma_matrix=SIG, device=dev, compute_stress=True,
+        skin=skin,
     )
     fn = om.MultiPairFn([1, 2], SIG, torch.ones(2, 2, dtype=F64), torch.full((2, 2), 2.0, dtype=F64))
     cpu = st.to("cpu")
@@ -60,9 +61,9 @@ def _cmp(g, o, tag, *, cell=True, tol=2e-6):
         assert v <= lim, (tag, k, v)
 
 
-@pytest.mark.parametrize("filt", ["frechet", "unit"])
-def test_ase_fire_cell_filter_matches_oracle(dev, filt):
-    st, model, model_fn = _ss(dev)
+@pytest.mark.parametrize("filt,skin", [("frechet", 0.0), ("unit", 0.0), ("frechet", 0.25)])
+def test_ase_fire_cell_filter_matches_oracle(dev, filt, skin):
+    st, model, model_fn = _ss(dev, skin=skin)
     cpu = st.to("cpu")
     g = tsa.fire_init(st, model, cell_filter=filt)
     o = of.fire_init(cpu.positions.clone(), cpu.cell.clone(), cpu.masses, cpu.system_idx, model_fn,

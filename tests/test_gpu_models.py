@@ -155,3 +155,29 @@ def test_non_periodic_molecule(dev):
                           compute_stress=False)
     out = model(st)
     _check(out, ref)
+
+
+def test_verlet_skin_matches_rebuild_every_call(dev):
+    """A model keeping a Verlet list with skin (gated per-system rebuilds,
+    strain-aware validity check) gives the forces, energies and stresses of a
+    list rebuilt at every call, through displacements and cell strains."""
+    st = tsa.workloads.soft_sphere_batch(6, n_min=64, n_max=200, device=dev, seed=21)
+    sig = torch.tensor([[1.0, 1.2], [1.2, 1.4]], dtype=F64)
+    kw = dict(atomic_numbers=torch.tensor([1, 2]), sigma_matrix=sig, device=dev, compute_stress=True)
+    m_skin = tsa.SoftSphereMultiModel(**kw, skin=0.3)
+    m_ref = tsa.SoftSphereMultiModel(**kw)
+    g = torch.Generator(device=dev).manual_seed(0)
+    x = st.clone()
+    for k in range(8):
+        if k:
+            x.positions = x.positions + 0.04 * torch.randn(x.positions.shape, generator=g,
+                                                           device=dev, dtype=F64)
+            strain = torch.eye(3, dtype=F64, device=dev) + 0.01 * torch.randn(
+                (x.n_systems, 3, 3), generator=g, device=dev, dtype=F64)
+            x.set_cell(torch.bmm(strain, x.cell), scale_atoms=True)
+        a, b = m_skin(x), m_ref(x)
+        torch.testing.assert_close(a["forces"], b["forces"], rtol=1e-12, atol=1e-12)
+        torch.testing.assert_close(a["energy"], b["energy"], rtol=1e-12, atol=1e-12)
+        torch.testing.assert_close(a["stress"], b["stress"], rtol=1e-12, atol=1e-12)
+    # one full build at the start, more only on capacity overflow
+    assert m_skin._vl.updates == 8 and m_skin._vl.full_builds <= 3

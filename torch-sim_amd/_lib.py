@@ -33,8 +33,10 @@ _SIGS = {
     "tsx_nl_ws_bytes": (L, [L, I, I]),
     "tsx_nl_build": (
         I,
-        [P, I, P, P, I, P, P, L, I, D, I, I, I, I, I, I, L, I, I, P, P, P, P, P, P, P, P, P],
+        [P, I, P, P, I, P, P, L, I, D, I, I, I, I, I, I, L, I, I, P, P, P, P, P, P, P, P, P, P],
     ),
+    "tsx_nl_check_ws_bytes": (L, [I]),
+    "tsx_nl_check": (I, [P, I, P, P, P, P, L, I, D, D, P, P, P]),
     "tsx_scan_ws_bytes": (L, [L]),
     "tsx_nl_count_pairs": (I, [P, L, P, P, P]),
     "tsx_nl_emit_pairs": (I, [P, P, P, P, P, P, L, I, I, L, P, L, P, P, P]),
@@ -44,7 +46,8 @@ _SIGS = {
         I,
         [I, P, I, P, P, I, P, P, P, P, P, P, L, I, I, L, D, I, P, P, P, P],
     ),
-    "tsx_system_reduce": (I, [P, P, P, P, P, L, I, P, P, P, P]),
+    "tsx_system_reduce": (I, [P, P, P, P, I, I, P, P, P, P, P]),
+    "tsx_reduce_ws_bytes": (L, [I, I]),
     "tsx_md_kick": (I, [P, P, D, L, P]),
     "tsx_md_drift": (I, [P, P, P, D, L, P]),
     "tsx_md_vv_half": (I, [P, P, P, P, D, L, P]),
@@ -56,7 +59,7 @@ _SIGS = {
             P, P, P, P, P, D, I, P, P, P, U, P, I, I, D, P, P,  # step part
         ],
     ),
-    "tsx_md_reduce": (I, [P, P, I, P, P, I, P, P]),
+    "tsx_md_reduce": (I, [P, P, I, I, P, P, I, P, P, P]),
     "tsx_copy_ref": (I, [P, P, L, P, P, P]),
     "tsx_fire_ws_bytes": (L, [I]),
     "tsx_fire_pre": (

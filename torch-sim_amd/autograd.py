@@ -28,20 +28,23 @@ import torch
 
 from . import _lib
 from ._lib import call, ptr, stream
-from .neighbors import NeighborList, SystemIndex
+from .neighbors import NeighborList, SystemIndex, VerletList
 
 
 class PairTerms:
     """The fixed part of a differentiable pair model (kind, species, cutoff)
     taken from a :class:`~torch_sim_amd.models.PairPotentialModel`."""
 
-    def __init__(self, model, *, tpa: int | None = None):
+    def __init__(self, model, *, tpa: int | None = None, skin: float = 0.0):
         self.model = model
         self.kind = model.kind
         self.n_species = model.n_species
         self.cutoff = model.cutoff
         self.tpa = tpa or model.tpa
-        self.nl = NeighborList(self.cutoff, tpa=self.tpa)
+        # skin > 0: a Verlet list kept between calls (trajectories rebuild only
+        # the systems that left the skin); every pair is still cut at cutoff
+        self.vl = VerletList(self.cutoff, skin, tpa=self.tpa) if skin > 0 else None
+        self.nl = NeighborList(self.cutoff, tpa=self.tpa) if self.vl is None else self.vl.nl
 
     def default_params(self) -> torch.Tensor:
         n = self.n_species
@@ -68,7 +71,10 @@ class Geometry:
     def rebuild(self, positions, cell) -> None:
         pos = positions.detach().to(torch.float64).contiguous()
         cl = cell.detach().to(torch.float64).contiguous()
-        self.terms.nl.build(pos, cl, self.pbc, self.sysidx)
+        if self.terms.vl is not None:
+            self.terms.vl.update(pos, cl, self.pbc, self.sysidx)
+        else:
+            self.terms.nl.build(pos, cl, self.pbc, self.sysidx)
 
     # raw kernel calls -----------------------------------------------------
     def forces(self, pos, cell, params) -> torch.Tensor:
@@ -95,7 +101,8 @@ class Geometry:
                  nl.jbits, nl.tpa, nl.ld, self.terms.cutoff, 0, ptr(f), ptr(e), None, stream())
         energy = torch.empty(s, dtype=torch.float64, device=pos.device)
         call("tsx_system_reduce", ptr(e), None, ptr(cell), ptr(self.sysidx.sys_start),
-             ptr(self.sysidx.atom_sys), n, s, ptr(energy), None, None, stream())
+             self.sysidx.max_atoms, s, ptr(energy), None, None,
+             ptr(self.sysidx.reduce_ws), stream())
         return energy
 
     def grad(self, pos, cell, params, *, w=None, ebar=None, want_pos=True, want_cell=False,

@@ -39,6 +39,16 @@ inline int blocks_for(int64_t n, int threads) {
   return (int)(b < 1 ? 1 : b);
 }
 
+// XCD-aware block order: consecutive hardware blocks are dealt round-robin
+// to the 8 XCDs of the chip (each with its own L2).  Renumbering so that
+// every XCD processes one contiguous range of rows keeps the neighbour
+// positions a block gathers in that XCD's L2.
+__device__ __forceinline__ int xcd_block() {
+  const int G = gridDim.x, b = blockIdx.x;
+  const int q = G >> 3, r = G & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 // ------------------------------------------------------ wave64 reductions
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -58,6 +68,17 @@ __device__ __forceinline__ double wave_max(double v) {
 __device__ __forceinline__ int wave_sum_int(int v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// inclusive scan of one int per lane over the wave
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
   return v;
 }
 

@@ -183,24 +183,6 @@ __global__ __launch_bounds__(256) void k_md_fused(FusedArgs A) {
 
 // per-system energy sums (fixed order), clear this step's rebuild gates and
 // advance the device step counter
-__global__ __launch_bounds__(256) void k_md_reduce(const double* __restrict__ e_atom,
-                                                   const int32_t* __restrict__ sys_start,
-                                                   double* __restrict__ energy,
-                                                   int64_t* __restrict__ step, int advance,
-                                                   int32_t* __restrict__ gate_clear) {
-  __shared__ double sh[4];
-  const int s = blockIdx.x;
-  const int a = sys_start[s], b = sys_start[s + 1];
-  double e = 0.0;
-  for (int k = a + threadIdx.x; k < b; k += blockDim.x) e += e_atom[k];
-  e = block_sum(e, sh);
-  if (threadIdx.x == 0) {
-    energy[s] = e;
-    if (gate_clear) gate_clear[s] = 0;
-    if (advance && s == 0) step[0] += 1;
-  }
-}
-
 // gated copy of the positions into the displacement reference
 // (counts the rebuilds that actually ran in ``counter`` when given)
 __global__ void k_copy_ref(const double* __restrict__ pos, double* __restrict__ xref, int64_t n4,
@@ -338,17 +320,6 @@ int tsx_md_fused_step(int32_t kind, const double* params, int32_t n_species,
   else
     rc = launch_fused_t<TSX_SOFT_SPHERE>(tpa, langevin != 0, vir, st, nblk, a);
   TSX_REQUIRE(rc == 0, "tpa must be 1, 2, 4 or 8");
-  TSX_CHECK_LAUNCH();
-  return TSX_OK;
-}
-
-int tsx_md_reduce(const double* e_atom, const int32_t* sys_start, int32_t n_systems,
-                  double* energy, int64_t* step, int32_t advance, int32_t* gate_clear,
-                  void* stream) {
-  TSX_REQUIRE(n_systems > 0, "n_systems must be positive");
-  TSX_REQUIRE(!advance || step, "step counter required to advance");
-  k_md_reduce<<<n_systems, 256, 0, (hipStream_t)stream>>>(e_atom, sys_start, energy, step,
-                                                           advance, gate_clear);
   TSX_CHECK_LAUNCH();
   return TSX_OK;
 }

@@ -20,6 +20,7 @@
 //               unwrapped-frame shift (torch_nl.py:214), ELL write with
 //               overflow detection.
 #include "tsx_common.h"
+#include "tsx_mat33.h"
 
 #include <cstdarg>
 
@@ -412,41 +413,56 @@ __global__ __launch_bounds__(256) void k_bin(
 }
 
 // ---------------------------------------------------------- per-system scan
-// One workgroup (256) per system: exclusive scan of its bin counts offset by
-// the system's first atom; bin_start[off + nbins] = sys_start[s + 1].
+// Exclusive scan of one system's bin counts into bin_start (one workgroup
+// per system).  Tiles of 4096 bins: each thread sums 16 consecutive counts,
+// a wave shuffle scan and four wave totals give the offsets, so a tile costs
+// two barriers (a 1M-atom box has ~4.4e5 bins).
+constexpr int SCAN_PER = 16;
+
 __global__ __launch_bounds__(256) void k_scan_sys(
     const int32_t* __restrict__ grid, const int32_t* __restrict__ meta, int bin_cap,
     const int32_t* __restrict__ sys_start, const int32_t* __restrict__ bin_count,
     int32_t* __restrict__ bin_start, const int32_t* __restrict__ gate,
-    int32_t* __restrict__ counter) {
-  __shared__ int32_t sh[256];
+    int32_t* __restrict__ counter, const double* __restrict__ cell_col,
+    double* __restrict__ cell_build) {
+  __shared__ int32_t wsum[4];
   __shared__ int32_t carry;
   const int s = blockIdx.x;
   if (meta[0] > bin_cap || gated_off(gate, s)) return;
   const int32_t* gr = grid + (size_t)GRID * s;
   const int off = gr[6], nb = gr[0] * gr[1] * gr[2];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (threadIdx.x == 0) carry = sys_start[s];
   __syncthreads();
-  for (int c0 = 0; c0 < nb; c0 += 256) {
-    const int c = c0 + threadIdx.x;
-    const int v = c < nb ? bin_count[off + c] : 0;
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int d = 1; d < 256; d <<= 1) {
-      const int t = threadIdx.x >= (unsigned)d ? sh[threadIdx.x - d] : 0;
-      __syncthreads();
-      sh[threadIdx.x] += t;
-      __syncthreads();
+  constexpr int TILE = 256 * SCAN_PER;
+  for (int c0 = 0; c0 < nb; c0 += TILE) {
+    const int base = c0 + threadIdx.x * SCAN_PER;
+    int v[SCAN_PER];
+    int tsum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+      v[k] = base + k < nb ? bin_count[off + base + k] : 0;
+      tsum += v[k];
     }
-    if (c < nb) bin_start[off + c] = carry + sh[threadIdx.x] - v;
+    const int incl = wave_incl_scan(tsum);
+    if (lane == 63) wsum[wid] = incl;
     __syncthreads();
-    if (threadIdx.x == 255) carry += sh[255];
+    int run = carry + incl - tsum;
+    for (int q = 0; q < wid; ++q) run += wsum[q];
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+      if (base + k < nb) bin_start[off + base + k] = run;
+      run += v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
     __syncthreads();
   }
   if (threadIdx.x == 0) {
     bin_start[off + nb] = sys_start[s + 1];
     if (counter) atomicAdd(counter, 1);
   }
+  if (cell_build && threadIdx.x < 9) cell_build[9 * s + threadIdx.x] = cell_col[9 * s + threadIdx.x];
 }
 
 __global__ __launch_bounds__(256) void k_scatter(
@@ -582,17 +598,6 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
-// inclusive scan of one int per lane over the wave
-__device__ __forceinline__ int wave_incl_scan(int v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int t = __shfl_up(v, d, 64);
-    if (lane >= d) v += t;
-  }
-  return v;
 }
 
 __global__ __launch_bounds__(64 * FILL_W) void k_fill(
@@ -772,6 +777,100 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
   }
 }
 
+// ---------------------------------------------------------- fill, per atom
+// One thread per list row t (atom order[t], in bin order).  The thread walks
+// the stencil images of its bin in the same order as the per-bin fill and the
+// atoms of each image bin in bin order, reading the wrapped coordinates from
+// L2 (neighbouring rows share bins, so the reads coalesce), and writes its
+// row in the order the candidates are met.  No LDS, no staging: sparse bins
+// (few atoms each) cost nothing beyond their candidates.
+__global__ __launch_bounds__(256) void k_fill_atom(
+    const int32_t* __restrict__ sys_start, const int32_t* __restrict__ atom_sys,
+    int64_t n_atoms, const double* __restrict__ geom, const int32_t* __restrict__ grid,
+    const int32_t* __restrict__ meta, int bin_cap, const int32_t* __restrict__ atom_bin,
+    const int32_t* __restrict__ bin_start, const int32_t* __restrict__ order,
+    const double* __restrict__ wr, double rl2, int self_int, int half, int jbits, int tpa,
+    int kmax, int64_t ld, uint32_t* __restrict__ nbr, int32_t* __restrict__ count,
+    int32_t* __restrict__ status, const int32_t* __restrict__ gate) {
+  const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  if (t >= n_atoms || meta[0] > bin_cap) return;
+  const int i = order[t];
+  const int s = atom_sys[i];
+  if (gated_off(gate, s)) return;
+  const double* g = geom + (size_t)GEOM * s;
+  const int32_t* gr = grid + (size_t)GRID * s;
+  const int base = sys_start[s];
+  const int off = gr[6];
+  const int wx = 2 * gr[3] + 1, wy = 2 * gr[4] + 1, wz = 2 * gr[5] + 1;
+  const int n_img = wx * wy * wz;
+  const int local = atom_bin[i] - off;
+  const int bz0 = local % gr[2], by0 = (local / gr[2]) % gr[1], bx0 = local / (gr[1] * gr[2]);
+  const int sbits = (32 - jbits) / 3;
+  const int bias = 1 << (sbits - 1);
+  const uint32_t fmask = (1u << sbits) - 1u;
+  const double4 wi = reinterpret_cast<const double4*>(wr)[t];
+  int w_i[3];
+  unpack_w(wi.w, w_i);
+  const double g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3], g4 = g[4], g5 = g[5], g6 = g[6],
+               g7 = g[7], g8 = g[8];
+  int cnt = 0;
+  bool shift_ovf = false;
+  for (int im = 0; im < n_img; ++im) {
+    const int dz = im % wz - gr[5];
+    const int dy = (im / wz) % wy - gr[4];
+    const int dx = im / (wy * wz) - gr[3];
+    const int cx = bx0 + dx, cy = by0 + dy, cz = bz0 + dz;
+    const int sx = floordiv(cx, gr[0]), sy = floordiv(cy, gr[1]), sz = floordiv(cz, gr[2]);
+    const int nbin =
+        off + ((cx - sx * gr[0]) * gr[1] + (cy - sy * gr[1])) * gr[2] + (cz - sz * gr[2]);
+    const int kb = bin_start[nbin], ke = bin_start[nbin + 1];
+    if (kb == ke) continue;
+    // S . L in the reference einsum order (transforms.py:563-565)
+    const double c0 = (double)sx * g0 + (double)sy * g3 + (double)sz * g6;
+    const double c1 = (double)sx * g1 + (double)sy * g4 + (double)sz * g7;
+    const double c2 = (double)sx * g2 + (double)sy * g5 + (double)sz * g8;
+    for (int k = kb; k < ke; ++k) {
+      const double4 wj = reinterpret_cast<const double4*>(wr)[k];
+      const double d2 = dist2_ref(wi.x, wi.y, wi.z, wj.x, wj.y, wj.z, c0, c1, c2);
+      if (!(d2 < rl2)) continue;
+      const int j = order[k];
+      bool match = !(j == i && (sx | sy | sz) == 0 && !self_int);
+      int w_j[3];
+      unpack_w(wj.w, w_j);
+      const int ux = sx + w_i[0] - w_j[0];
+      const int uy = sy + w_i[1] - w_j[1];
+      const int uz = sz + w_i[2] - w_j[2];
+      if (match && half) {
+        if (i != j) {
+          match = i < j;
+        } else {
+          const int first = ux != 0 ? ux : (uy != 0 ? uy : uz);
+          match = first > 0;
+        }
+      }
+      if (match && (ux < -bias || ux >= bias || uy < -bias || uy >= bias || uz < -bias ||
+                    uz >= bias)) {
+        shift_ovf = true;
+        match = false;
+      }
+      if (!match) continue;
+      if (cnt < kmax)
+        nbr[ell_index(t, cnt, tpa, ld)] =
+            (uint32_t)(j - base) |
+            ((((uint32_t)(ux + bias) & fmask) | (((uint32_t)(uy + bias) & fmask) << sbits) |
+              (((uint32_t)(uz + bias) & fmask) << (2 * sbits)))
+             << jbits);
+      ++cnt;
+    }
+  }
+  if (cnt > kmax) {
+    atomicOr(status, TSX_STATUS_NL_OVERFLOW);
+    cnt = kmax;
+  }
+  if (shift_ovf) atomicOr(status, TSX_STATUS_SHIFT_OVERFLOW);
+  count[t] = cnt;
+}
+
 __global__ void k_emit(const uint32_t* __restrict__ nbr, const int32_t* __restrict__ count,
                        const int32_t* __restrict__ order, const int64_t* __restrict__ row_ptr,
                        const int32_t* __restrict__ sys_start, const int32_t* __restrict__ atom_sys,
@@ -814,11 +913,82 @@ __global__ void k_wrap_out(const double* __restrict__ pos, const double* __restr
   }
 }
 
+// ------------------------------------------------------- Verlet skin check
+// A list built at cell Lb with radius r_list stays valid for cutoff rc while
+// every atom, pulled back to the build cell (Lb L^-1 r), is within thr of its
+// position at the build: pairs outside the list were >= r_list apart then,
+// so they are >= sigma (r_list - 2 thr) >= rc apart now, where
+//   thr = (r_list - rc / sigma) / 2,   sigma = 1 - ||L Lb^-1 - I||_F
+// bounds the smallest singular value of the cell map from below.
+constexpr int CHECK_W = 10;  // per-system workspace: pull-back [9], thr^2
+
+__global__ void k_check_sys(const double* __restrict__ cell_col,
+                            const double* __restrict__ cell_build, int n_sys, double r_list,
+                            double cutoff, double* __restrict__ ws, int32_t* __restrict__ flags) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_sys) return;
+  const M33 cur = m33_load(cell_col + 9 * (int64_t)s);
+  const M33 bld = m33_load(cell_build + 9 * (int64_t)s);
+  const M33 a = m33_mul(cur, m33_inv(bld));
+  const double sig = 1.0 - m33_frob(m33_sub(a, m33_eye()));
+  double* w = ws + (int64_t)CHECK_W * s;
+  m33_store(m33_mul(bld, m33_inv(cur)), w);
+  int flag = 0;
+  if (!(sig > 0.0) || !(sig * r_list > cutoff)) {
+    w[9] = -1.0;
+    flag = 1;
+  } else {
+    const double thr = 0.5 * (r_list - cutoff / sig);
+    w[9] = thr * thr;
+  }
+  flags[s] = flag;
+}
+
+__global__ void k_check_atoms(const double* __restrict__ pos, int ps,
+                              const double* __restrict__ xref,
+                              const int32_t* __restrict__ atom_sys, int64_t n,
+                              const double* __restrict__ ws, int32_t* __restrict__ flags) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = atom_sys[i];
+  const double* w = ws + (int64_t)CHECK_W * s;
+  const double* r = pos + (int64_t)ps * i;
+  double rt[3];
+  m33_apply(m33_load(w), r, rt);
+  const double4 x0 = reinterpret_cast<const double4*>(xref)[i];
+  const double dx = rt[0] - x0.x, dy = rt[1] - x0.y, dz = rt[2] - x0.z;
+  if (!(dx * dx + dy * dy + dz * dz <= w[9])) flags[s] = 1;
+}
+
 }  // namespace tsx
 
 using namespace tsx;
 
 extern "C" {
+
+int64_t tsx_nl_check_ws_bytes(int32_t n_systems) {
+  return (int64_t)sizeof(double) * CHECK_W * (n_systems > 0 ? n_systems : 1);
+}
+
+int tsx_nl_check(const double* pos, int32_t pos_stride, const double* cell_col,
+                 const double* cell_build, const double* xref, const int32_t* atom_sys,
+                 int64_t n_atoms, int32_t n_systems, double r_list, double cutoff,
+                 int32_t* flags, void* ws, void* stream) {
+  TSX_REQUIRE(n_systems > 0, "n_systems must be positive");
+  TSX_REQUIRE(pos_stride == 3 || pos_stride == 4, "pos_stride must be 3 or 4");
+  TSX_REQUIRE(r_list > cutoff && cutoff > 0.0, "need r_list > cutoff > 0");
+  TSX_REQUIRE(ws && flags && cell_build && xref, "ws, flags, cell_build and xref required");
+  hipStream_t st = (hipStream_t)stream;
+  k_check_sys<<<blocks_for(n_systems, 64), 64, 0, st>>>(cell_col, cell_build, n_systems, r_list,
+                                                        cutoff, (double*)ws, flags);
+  TSX_CHECK_LAUNCH();
+  if (n_atoms > 0) {
+    k_check_atoms<<<blocks_for(n_atoms, 256), 256, 0, st>>>(pos, pos_stride, xref, atom_sys,
+                                                            n_atoms, (const double*)ws, flags);
+    TSX_CHECK_LAUNCH();
+  }
+  return TSX_OK;
+}
 
 int tsx_version(void) { return 100; }
 
@@ -834,7 +1004,7 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
                  int32_t bins_per_rlist, int32_t self_interaction, int32_t half, int32_t jbits,
                  int32_t tpa, int32_t kmax, int64_t ld, int32_t bin_cap, int32_t setup, void* ws,
                  uint32_t* nbr, int32_t* count, int32_t* order, double* xref, int32_t* status,
-                 const int32_t* gate, int32_t* counter, void* stream) {
+                 const int32_t* gate, int32_t* counter, double* cell_build, void* stream) {
   TSX_REQUIRE(n_systems > 0, "n_systems must be positive");
   TSX_REQUIRE(n_atoms >= 0 && n_atoms < (1ll << 31), "n_atoms out of range");
   TSX_REQUIRE(pos_stride == 3 || pos_stride == 4, "pos_stride must be 3 or 4");
@@ -865,7 +1035,7 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
                                              w.atom_bin, w.slot, gate, xref);
   TSX_CHECK_LAUNCH();
   k_scan_sys<<<n_systems, 256, 0, st>>>(w.grid, w.meta, bin_cap, sys_start, w.bin_count,
-                                        w.bin_start, gate, counter);
+                                        w.bin_start, gate, counter, cell_col, cell_build);
   TSX_CHECK_LAUNCH();
   k_scatter<<<n_systems * b_atoms, 256, 0, st>>>(sys_start, n_systems, b_atoms, w.grid, w.meta,
                                                  bin_cap, w.atom_bin, w.slot, w.bin_start, order,
@@ -879,11 +1049,10 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
                                               w.meta, bin_cap, w.act, w.bin_start, w.bin_count,
                                               order, w.wr);
   TSX_CHECK_LAUNCH();
-  const int g_fill = blocks_for(bin_cap, FILL_W) < 768 ? blocks_for(bin_cap, FILL_W) : 768;
-  k_fill<<<g_fill, 64 * FILL_W, 0, st>>>(sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap,
-                                         w.act, w.bin_start, order, w.wr, r_list * r_list,
-                                         self_interaction, half, jbits, tpa, kmax, ld, nbr,
-                                         count, status);
+  k_fill_atom<<<blocks_for(n_atoms, 256), 256, 0, st>>>(
+      sys_start, atom_sys, n_atoms, w.geom, w.grid, w.meta, bin_cap, w.atom_bin, w.bin_start,
+      order, w.wr, r_list * r_list, self_interaction, half, jbits, tpa, kmax, ld, nbr, count,
+      status, gate);
   TSX_CHECK_LAUNCH();
   return TSX_OK;
 }

@@ -223,14 +223,5 @@ __device__ __forceinline__ RowAcc accumulate_row(
   return a;
 }
 
-// XCD-aware block order: consecutive hardware blocks are dealt round-robin
-// to the 8 XCDs of the chip (each with its own L2).  Renumbering so that
-// every XCD processes one contiguous range of rows keeps the neighbour
-// positions a block gathers in that XCD's L2.
-__device__ __forceinline__ int xcd_block() {
-  const int G = gridDim.x, b = blockIdx.x;
-  const int q = G >> 3, r = G & 7, x = b & 7;
-  return x * q + (x < r ? x : r) + (b >> 3);
-}
 
 }  // namespace tsx

@@ -174,52 +174,6 @@ __global__ void k_zero(double* __restrict__ p, int64_t n) {
   if (k < n) p[k] = 0.0;
 }
 
-// One workgroup per system: fixed-order sums of the per-atom energies and
-// virials (deterministic), stress = -W / |det L| (pair_potential.py:200-274).
-__global__ __launch_bounds__(256) void k_system_reduce(
-    const double* __restrict__ e_atom, const double* __restrict__ w_atom,
-    const double* __restrict__ cell_col, const int32_t* __restrict__ sys_start,
-    double* __restrict__ energy, double* __restrict__ stress, double* __restrict__ stresses) {
-  __shared__ double sh[4];
-  const int s = blockIdx.x;
-  const int a = sys_start[s], b = sys_start[s + 1];
-  double e = 0.0, w[6] = {0, 0, 0, 0, 0, 0};
-  for (int k = a + threadIdx.x; k < b; k += blockDim.x) {
-    e += e_atom[k];
-    if (w_atom)
-      for (int c = 0; c < 6; ++c) w[c] += w_atom[6 * (int64_t)k + c];
-  }
-  e = block_sum(e, sh);
-  if (threadIdx.x == 0) energy[s] = e;
-  if (!w_atom) return;
-  double C[9];
-  for (int k = 0; k < 9; ++k) C[k] = cell_col[9 * (int64_t)s + k];
-  const double vol = fabs(det33(C));
-  const double iv = 1.0 / vol;
-  for (int c = 0; c < 6; ++c) w[c] = block_sum(w[c], sh);
-  if (stress && threadIdx.x == 0) {
-    double* o = stress + 9 * (int64_t)s;
-    o[0] = -w[0] / vol;
-    o[4] = -w[1] / vol;
-    o[8] = -w[2] / vol;
-    o[5] = o[7] = -w[3] / vol;
-    o[2] = o[6] = -w[4] / vol;
-    o[1] = o[3] = -w[5] / vol;
-  }
-  if (stresses) {
-    for (int k = a + threadIdx.x; k < b; k += blockDim.x) {
-      const double* wk = w_atom + 6 * (int64_t)k;
-      double* o = stresses + 9 * (int64_t)k;
-      o[0] = -wk[0] * iv;
-      o[4] = -wk[1] * iv;
-      o[8] = -wk[2] * iv;
-      o[5] = o[7] = -wk[3] * iv;
-      o[2] = o[6] = -wk[4] * iv;
-      o[1] = o[3] = -wk[5] * iv;
-    }
-  }
-}
-
 template <int KIND, int TPA, bool HALF, bool VIRIAL, int PS>
 static void launch_pair(hipStream_t st, int nblk, const double* params, int ns,
                         const int32_t* species, const double* pos, const double* cell,
@@ -330,20 +284,6 @@ int tsx_pair_forces(int32_t kind, const double* params, int32_t n_species,
     rc = dispatch_tpa<TSX_SOFT_SPHERE>(TSX_ARGS);
 #undef TSX_ARGS
   TSX_REQUIRE(rc == 0, "unsupported launch configuration");
-  TSX_CHECK_LAUNCH();
-  return TSX_OK;
-}
-
-int tsx_system_reduce(const double* e_atom, const double* w_atom, const double* cell_col,
-                      const int32_t* sys_start, const int32_t* atom_sys, int64_t n_atoms,
-                      int32_t n_systems, double* energy, double* stress, double* stresses,
-                      void* stream) {
-  TSX_REQUIRE(n_systems > 0, "n_systems must be positive");
-  (void)atom_sys;
-  (void)n_atoms;
-  hipStream_t st = (hipStream_t)stream;
-  k_system_reduce<<<n_systems, 256, 0, st>>>(e_atom, w_atom, cell_col, sys_start, energy,
-                                              stress, stresses);
   TSX_CHECK_LAUNCH();
   return TSX_OK;
 }

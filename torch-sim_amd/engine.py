@@ -91,6 +91,7 @@ class BatchedMD:
         self.device = dev
         self.state_template = state
         self.sysidx = SystemIndex(state.system_idx, s)
+        self.sysidx.reduce_ws  # allocated (zeroed) before any graph capture
         self.cell = state.cell.detach().to(torch.float64).contiguous()
         pbc = state.pbc
         self.pbc_per_system = pbc.ndim == 2
@@ -148,8 +149,9 @@ class BatchedMD:
 
     def _reduce(self, q: int) -> None:
         call(
-            "tsx_md_reduce", ptr(self.e_atom), ptr(self.sysidx.sys_start), self.n_systems,
-            ptr(self.energy), ptr(self.step_ctr), 1, ptr(self._gate(q)), stream(),
+            "tsx_md_reduce", ptr(self.e_atom), ptr(self.sysidx.sys_start),
+            self.sysidx.max_atoms, self.n_systems, ptr(self.energy), ptr(self.step_ctr), 1,
+            ptr(self._gate(q)), ptr(self.sysidx.reduce_ws), stream(),
         )
 
     def _rebuild(self, q: int) -> None:

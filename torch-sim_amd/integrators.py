@@ -86,8 +86,9 @@ def initialize_momenta(
     ) * torch.sqrt(masses * kT).unsqueeze(-1)
     n_sys = int(system_idx[-1]) + 1
     cnt = torch.bincount(system_idx, minlength=n_sys)
-    mean = torch.zeros((n_sys, 3), dtype=p.dtype, device=p.device).index_add(0, system_idx, p)
-    mean = mean / cnt.view(-1, 1)
+    # segmented sum over the contiguous systems (an index_add would serialise
+    # a single large system on one address)
+    mean = torch.segment_reduce(p, "sum", lengths=cnt, axis=0) / cnt.view(-1, 1)
     return torch.where(
         torch.repeat_interleave(cnt > 1, cnt).view(-1, 1), p - mean[system_idx], p
     )

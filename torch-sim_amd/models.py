@@ -24,7 +24,7 @@ import torch
 
 from . import _lib
 from ._lib import call, ptr, stream
-from .neighbors import NeighborList, SystemIndex, torchsim_nl
+from .neighbors import VerletList, NeighborList, SystemIndex, torchsim_nl
 
 LJ, MORSE, SOFT_SPHERE = 0, 1, 2
 
@@ -124,6 +124,7 @@ class PairPotentialModel(ModelInterface):
         reduce_to_half_list: bool = True,
         retain_graph: bool = False,
         tpa: int = 4,
+        skin: float = 0.0,
     ) -> None:
         super().__init__()
         self._device = _default_device(device)
@@ -173,7 +174,13 @@ class PairPotentialModel(ModelInterface):
                 table[int(zz)] = k
             self._z_to_idx = table.to(self._device)
         self._nl = NeighborList(self.cutoff, tpa=tpa)
+        # skin > 0: keep a Verlet list between calls and rebuild only the systems
+        # that left the skin (results equal a rebuild at every call)
+        self.skin = float(skin)
+        self._vl = VerletList(self.cutoff, self.skin, tpa=tpa) if self.skin > 0 else None
         self._sysidx: SystemIndex | None = None
+        self._species_key = None
+        self._species = None
         self.tpa = tpa
 
     # ------------------------------------------------------------- helpers
@@ -190,11 +197,15 @@ class PairPotentialModel(ModelInterface):
         their parameter table with it)."""
         if self.n_species == 1:
             return None
+        key = (atomic_numbers.data_ptr(), atomic_numbers._version, atomic_numbers.shape[0])
+        if key == self._species_key:
+            return self._species
         if int(atomic_numbers.max()) >= self._z_to_idx.numel() or int(atomic_numbers.min()) < 0:
             raise ValueError("atomic number outside the model's species table")
         sp = self._z_to_idx[atomic_numbers].contiguous()
         if int(sp.min()) < 0:
             raise ValueError("state contains an atomic number that is not a model species")
+        self._species_key, self._species = key, sp
         return sp
 
     def forward(self, state, **_kwargs) -> dict[str, torch.Tensor]:
@@ -214,7 +225,11 @@ class PairPotentialModel(ModelInterface):
         n, s = state.n_atoms, state.n_systems
         si = self._system_index(state)
         pbc = state.pbc
-        self._nl.build(pos, cell, pbc, si)
+        if self._vl is not None:
+            nl = self._vl.update(pos, cell, pbc, si)
+        else:
+            nl = self._nl
+            nl.build(pos, cell, pbc, si)
         dev = pos.device
         forces = torch.empty((n, 3), dtype=torch.float64, device=dev)
         e_atom = torch.empty(n, dtype=torch.float64, device=dev)
@@ -225,9 +240,9 @@ class PairPotentialModel(ModelInterface):
             call(
                 "tsx_pair_forces",
                 self.kind, ptr(self._params), self.n_species, ptr(species), ptr(pos), 3,
-                ptr(cell), ptr(si.sys_start), ptr(si.atom_sys), ptr(self._nl.nbr),
-                ptr(self._nl.count), ptr(self._nl.order), n, self._nl.jbits, self._nl.tpa,
-                self._nl.ld, self.cutoff, 0, ptr(forces), ptr(e_atom), ptr(w_atom), stream(),
+                ptr(cell), ptr(si.sys_start), ptr(si.atom_sys), ptr(nl.nbr), ptr(nl.count),
+                ptr(nl.order), n, nl.jbits, nl.tpa, nl.ld, self.cutoff, 0, ptr(forces),
+                ptr(e_atom), ptr(w_atom), stream(),
             )
         energy = torch.empty(s, dtype=torch.float64, device=dev)
         stress = torch.empty((s, 3, 3), dtype=torch.float64, device=dev) if need_w else None
@@ -238,7 +253,8 @@ class PairPotentialModel(ModelInterface):
         )
         call(
             "tsx_system_reduce", ptr(e_atom), ptr(w_atom), ptr(cell), ptr(si.sys_start),
-            ptr(si.atom_sys), n, s, ptr(energy), ptr(stress), ptr(stresses), stream(),
+            si.max_atoms, s, ptr(energy), ptr(stress), ptr(stresses), ptr(si.reduce_ws),
+            stream(),
         )
         dt = self._dtype
         out = {"energy": energy.to(dt)}
@@ -271,6 +287,7 @@ class LennardJonesModel(PairPotentialModel):
         cutoff: float | None = None,
         retain_graph: bool = False,
         tpa: int = 4,
+        skin: float = 0.0,
     ) -> None:
         self.sigma = sigma
         self.epsilon = epsilon
@@ -288,6 +305,7 @@ class LennardJonesModel(PairPotentialModel):
             neighbor_list_fn=neighbor_list_fn,
             retain_graph=retain_graph,
             tpa=tpa,
+            skin=skin,
         )
 
 
@@ -310,6 +328,7 @@ class MorseModel(PairPotentialModel):
         cutoff: float | None = None,
         retain_graph: bool = False,
         tpa: int = 4,
+        skin: float = 0.0,
     ) -> None:
         self.sigma = sigma
         self.epsilon = epsilon
@@ -329,6 +348,7 @@ class MorseModel(PairPotentialModel):
             neighbor_list_fn=neighbor_list_fn,
             retain_graph=retain_graph,
             tpa=tpa,
+            skin=skin,
         )
 
 
@@ -352,6 +372,7 @@ class SoftSphereModel(PairPotentialModel):
         cutoff: float | None = None,
         retain_graph: bool = False,
         tpa: int = 4,
+        skin: float = 0.0,
     ) -> None:
         self.sigma = sigma
         self.epsilon = epsilon
@@ -371,6 +392,7 @@ class SoftSphereModel(PairPotentialModel):
             neighbor_list_fn=neighbor_list_fn,
             retain_graph=retain_graph,
             tpa=tpa,
+            skin=skin,
         )
 
 
@@ -396,6 +418,7 @@ class SoftSphereMultiModel(PairPotentialModel):
         cutoff: float | None = None,
         retain_graph: bool = False,
         tpa: int = 4,
+        skin: float = 0.0,
     ) -> None:
         self.pbc = torch.tensor([pbc] * 3) if isinstance(pbc, bool) else pbc
         self.use_neighbor_list = use_neighbor_list
@@ -425,6 +448,7 @@ class SoftSphereMultiModel(PairPotentialModel):
             neighbor_list_fn=neighbor_list_fn,
             retain_graph=retain_graph,
             tpa=tpa,
+            skin=skin,
         )
 
 
@@ -450,6 +474,7 @@ class MultiLennardJonesModel(PairPotentialModel):
         per_atom_energies: bool = False,
         per_atom_stresses: bool = False,
         tpa: int = 4,
+        skin: float = 0.0,
     ) -> None:
         unique_z = torch.unique(torch.as_tensor(atomic_numbers).cpu()).sort().values.long()
         self.sigma_matrix = sigma_matrix.detach().cpu().double()
@@ -468,4 +493,5 @@ class MultiLennardJonesModel(PairPotentialModel):
             per_atom_energies=per_atom_energies,
             per_atom_stresses=per_atom_stresses,
             tpa=tpa,
+            skin=skin,
         )

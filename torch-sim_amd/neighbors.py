@@ -46,6 +46,23 @@ class SystemIndex:
         self.sys_start = torch.searchsorted(system_idx, bounds).to(torch.int32).contiguous()
         self._key = (system_idx.data_ptr(), system_idx._version, self.n_atoms, self.n_systems)
         self._counts = None
+        self._red_ws = None
+
+    @property
+    def max_atoms(self) -> int:
+        """Atoms of the largest system (host, cached)."""
+        c = self.counts
+        return int(c.max()) if c.size else 0
+
+    @property
+    def reduce_ws(self) -> torch.Tensor:
+        """Zeroed workspace of the per-system reductions (tsx_system_reduce /
+        tsx_md_reduce); reductions that share it run on one stream."""
+        if self._red_ws is None:
+            nbytes = int(_lib.lib().tsx_reduce_ws_bytes(self.max_atoms, self.n_systems))
+            self._red_ws = torch.zeros((nbytes + 7) // 8, dtype=torch.float64,
+                                       device=self.atom_sys.device)
+        return self._red_ws
 
     def matches(self, system_idx: torch.Tensor, n_systems: int) -> bool:
         return self._key == (
@@ -187,6 +204,7 @@ class NeighborList:
         gate: torch.Tensor | None = None,
         xref: torch.Tensor | None = None,
         counter: torch.Tensor | None = None,
+        cell_build: torch.Tensor | None = None,
     ) -> None:
         """Enqueue a build on the current stream (capturable, no host sync).
 
@@ -199,7 +217,7 @@ class NeighborList:
             self.r_list, self.bins_per_rlist, int(self.self_interaction), int(self.half),
             self.jbits, self.tpa, self.kmax, self.ld, self.bin_cap, int(setup), ptr(self.ws),
             ptr(self.nbr), ptr(self.count), ptr(self.order), ptr(xref), ptr(self.status),
-            ptr(gate), ptr(counter), stream(),
+            ptr(gate), ptr(counter), ptr(cell_build), stream(),
         )
 
     def build(
@@ -211,6 +229,7 @@ class NeighborList:
         pos_stride: int = 3,
         max_retries: int = 12,
         xref: torch.Tensor | None = None,
+        cell_build: torch.Tensor | None = None,
     ) -> None:
         """Build with host-side capacity planning and overflow retries."""
         _lib.require_gpu(pos, "positions")
@@ -230,7 +249,8 @@ class NeighborList:
         self.jbits = jbits
         for _ in range(max_retries):
             self.status.zero_()
-            self.enqueue(pos, pos_stride, cell_col, pbc_u8, pbc_per_system, sysidx, xref=xref)
+            self.enqueue(pos, pos_stride, cell_col, pbc_u8, pbc_per_system, sysidx, xref=xref,
+                         cell_build=cell_build)
             st = int(self.status.item())
             if st & STATUS_SINGULAR_CELL:
                 raise _lib.TsxError("periodic system with a singular cell")
@@ -272,6 +292,75 @@ class NeighborList:
                 ptr(mapping), n_pairs, ptr(sysmap), ptr(shifts), stream(),
             )
         return mapping, sysmap, shifts.to(dtype)
+
+
+class VerletList:
+    """A neighbor list with skin kept across calls.
+
+    ``update`` rebuilds only the systems whose atoms, pulled back to the cell
+    of their last build, moved out of the skin or whose cell strained past
+    what the skin absorbs (``tsx_nl_check`` then a gated ``tsx_nl_build``);
+    every pair is still tested against the cutoff in the force kernels, so
+    results equal a list rebuilt at every call (the reference behaviour,
+    neighbors/__init__.py:46-98).  One host read of the build status per
+    update (capacity overflow falls back to a full build with more room).
+    """
+
+    def __init__(self, cutoff: float, skin: float, *, tpa: int = 4, bins_per_rlist: int = 2,
+                 self_interaction: bool = False):
+        if skin <= 0:
+            raise ValueError("skin must be positive")
+        self.cutoff = float(cutoff)
+        self.skin = float(skin)
+        self.nl = NeighborList(self.cutoff + self.skin, self_interaction=self_interaction,
+                               tpa=tpa, bins_per_rlist=bins_per_rlist)
+        self._key = None
+        self.xref = self.cell_build = self.flags = self.ws = None
+        self.full_builds = 0
+        self.updates = 0
+
+    def _full(self, pos, cell_col, pbc, sysidx, pos_stride):
+        dev = pos.device
+        self.xref = torch.zeros((max(1, sysidx.n_atoms), 4), dtype=torch.float64, device=dev)
+        self.cell_build = torch.zeros((sysidx.n_systems, 3, 3), dtype=torch.float64, device=dev)
+        self.flags = torch.zeros(sysidx.n_systems, dtype=torch.int32, device=dev)
+        nbytes = int(_lib.lib().tsx_nl_check_ws_bytes(sysidx.n_systems))
+        self.ws = torch.empty((nbytes + 7) // 8, dtype=torch.float64, device=dev)
+        self.nl.build(pos, cell_col, pbc, sysidx, pos_stride, xref=self.xref,
+                      cell_build=self.cell_build)
+        self._key = (sysidx._key, sysidx.n_atoms, sysidx.n_systems)
+        self.full_builds += 1
+
+    def update(self, pos: torch.Tensor, cell_col: torch.Tensor, pbc: torch.Tensor,
+               sysidx: SystemIndex, pos_stride: int = 3) -> NeighborList:
+        """Make the list valid for ``pos`` / ``cell_col``; returns it."""
+        self.updates += 1
+        key = (sysidx._key, sysidx.n_atoms, sysidx.n_systems)
+        if self._key != key or self.xref is None:
+            self._full(pos, cell_col, pbc, sysidx, pos_stride)
+            return self.nl
+        if sysidx.n_atoms == 0:
+            return self.nl
+        call("tsx_nl_check", ptr(pos), pos_stride, ptr(cell_col), ptr(self.cell_build),
+             ptr(self.xref), ptr(sysidx.atom_sys), sysidx.n_atoms, sysidx.n_systems,
+             self.nl.r_list, self.cutoff, ptr(self.flags), ptr(self.ws), stream())
+        self.nl.status.zero_()
+        self.nl.enqueue(pos, pos_stride, cell_col, pbc.to(torch.uint8).contiguous(), pbc.ndim == 2,
+                        sysidx, setup=True, gate=self.flags, xref=self.xref,
+                        cell_build=self.cell_build)
+        st = int(self.nl.status.item())
+        if st & STATUS_SINGULAR_CELL:
+            raise _lib.TsxError("periodic system with a singular cell")
+        if st & (STATUS_NL_OVERFLOW | STATUS_BIN_OVERFLOW | STATUS_SHIFT_OVERFLOW):
+            # more room (or wrapped images) needed: full build of every system
+            if st & STATUS_NL_OVERFLOW:
+                self.nl.ensure(sysidx.n_atoms, sysidx.n_systems, self.nl.kmax * 2,
+                               self.nl.bin_cap, self.nl.jbits, pos.device)
+            if st & STATUS_BIN_OVERFLOW:
+                self.nl.ensure(sysidx.n_atoms, sysidx.n_systems, self.nl.kmax,
+                               self.nl.bin_cap * 4, self.nl.jbits, pos.device)
+            self._full(pos, cell_col, pbc, sysidx, pos_stride)
+        return self.nl
 
 
 def _normalize(cell: torch.Tensor, pbc: torch.Tensor, n_systems: int):
