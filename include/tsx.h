@@ -109,7 +109,17 @@ const char* tsx_last_error(void);
  * system.  ``xref`` (stride 4, may be NULL) receives a copy of the positions
  * of the rebuilt systems (the displacement reference of the skin) and
  * ``cell_build`` ([S,3,3], may be NULL) a copy of their cells.
+ *
+ * ``fill_mode``: TSX_FILL_BIN stages each bin's stencil candidates in LDS
+ * (one wave per bin: dense lists, partial rebuilds of many systems);
+ * TSX_FILL_ATOM gives each atom a group of lanes reading its stencil columns
+ * from L2 (sparse bins, full builds of large systems); TSX_FILL_AUTO picks
+ * BIN for bins_per_rlist >= 2, kmax >= 64 and more than one system.  Both
+ * write every row in the same order (stencil image, then atom index).
  */
+#define TSX_FILL_AUTO 0
+#define TSX_FILL_BIN 1
+#define TSX_FILL_ATOM 2
 int64_t tsx_nl_ws_bytes(int64_t n_atoms, int32_t n_systems, int32_t bin_cap);
 
 int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
@@ -119,7 +129,7 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
                  int32_t half, int32_t jbits, int32_t tpa, int32_t kmax, int64_t ld,
                  int32_t bin_cap, int32_t setup, void* ws, uint32_t* nbr, int32_t* count,
                  int32_t* order, double* xref, int32_t* status, const int32_t* gate,
-                 int32_t* counter, double* cell_build, void* stream);
+                 int32_t* counter, double* cell_build, int32_t fill_mode, void* stream);
 
 /* Exclusive scan of ``count`` into ``row_ptr`` [n+1] (int64); the total is
  * row_ptr[n].  Used before tsx_nl_emit_pairs.  ``ws`` holds
@@ -194,9 +204,7 @@ int tsx_system_reduce(const double* e_atom, const double* w_atom, const double* 
                       void* stream);
 
 /* Workspace bytes for tsx_system_reduce / tsx_md_reduce (max_system_atoms =
- * the largest system).  The workspace must be zeroed once before its first
- * use; every call leaves its counter at zero again (so calls can be captured
- * in a graph and replayed). */
+ * the largest system): chunk partials of systems above 2048 atoms. */
 int64_t tsx_reduce_ws_bytes(int32_t max_system_atoms, int32_t n_systems);
 
 

@@ -51,3 +51,43 @@ def test_binding_arity_matches_header():
     bad = {n: (c, len(tsa._lib._SIGS[n][1])) for n, c in counts.items()
            if len(tsa._lib._SIGS[n][1]) != c}
     assert not bad, bad
+
+
+def _param_types():
+    """ctypes code of every parameter of every header declaration."""
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for m in re.finditer(r"^(?:int|int64_t|const char\*)\s+(tsx_\w+)\(([^;]*)\);", text, re.M | re.S):
+        args = m.group(2).strip()
+        codes = []
+        if args not in ("", "void"):
+            for a in args.split(","):
+                a = " ".join(a.split())
+                if "*" in a:
+                    codes.append("P")
+                elif a.startswith("int32_t") or a.startswith("int "):
+                    codes.append("I")
+                elif a.startswith("int64_t"):
+                    codes.append("L")
+                elif a.startswith("uint64_t"):
+                    codes.append("U")
+                elif a.startswith("double"):
+                    codes.append("D")
+                else:
+                    codes.append("?" + a)
+        out[m.group(1)] = codes
+    return out
+
+
+def test_binding_types_match_header():
+    import ctypes
+
+    code = {ctypes.c_void_p: "P", ctypes.c_int32: "I", ctypes.c_int64: "L", ctypes.c_uint64: "U",
+            ctypes.c_double: "D"}
+    bad = {}
+    for name, want in _param_types().items():
+        got = [code.get(t, "?") for t in tsa._lib._SIGS[name][1]]
+        if got != want:
+            bad[name] = (want, got)
+    assert not bad, bad

@@ -117,6 +117,27 @@ def test_layouts_and_bin_sizes_give_same_set(dev, tpa, q):
     _assert_same(onl.sort_pairs(*(t.cpu() for t in nl.pairs(si, F64))), ref)
 
 
+@pytest.mark.parametrize("q", [1, 2])
+def test_fill_kernels_write_identical_rows(dev, q):
+    """The per-bin (LDS-staged) and per-atom fill kernels write every row with
+    the same neighbours in the same order (stencil image, then atom index), on
+    small triclinic cells (many periodic images) and larger ones."""
+    st = tsa.workloads.cu_batch(6, device="cpu", seed=8)
+    small = tsa.workloads.cu_batch(3, device="cpu", reps=[[2, 2, 2], [1, 1, 1], [1, 2, 3]], seed=2)
+    tri = tsa.workloads.soft_sphere_batch(4, n_min=40, n_max=120, device="cpu", seed=3)
+    for s_ in (st, small, tri):
+        si = SystemIndex(s_.system_idx.cuda(), s_.n_systems)
+        outs = []
+        for mode in (1, 2):
+            nl = NeighborList(6.0, tpa=4, bins_per_rlist=q, fill_mode=mode)
+            nl.build(s_.positions.cuda(), s_.cell.cuda().contiguous(), s_.pbc.cuda(), si)
+            outs.append([t.cpu() for t in nl.pairs(si, F64)])
+        for a, b in zip(*outs):
+            assert torch.equal(a, b)
+        ref = onl.brute_force_nl(s_.positions, s_.cell.mT, s_.pbc, 6.0, s_.system_idx)
+        _assert_same(onl.sort_pairs(*outs[0]), ref)
+
+
 def test_wrap_positions_matches_oracle(dev):
     g = GOLD["wrap_batched"]
     cell = torch.tensor(g["cell_row"], dtype=F64)

@@ -54,6 +54,7 @@ struct NLWs {
   int32_t* slot;       // N
   double* wr;          // N x 4 (wrapped xyz, packed wrap shift)
   int32_t* scan_tmp;   // scan block sums
+  int32_t* unsorted;   // N: atoms scattered to their bins, before the in-bin sort
   int32_t* act;        // [S + 1] exclusive prefix of the bins of gated-on systems
 };
 
@@ -76,6 +77,7 @@ static size_t ws_layout(int64_t n, int32_t s, int32_t bin_cap, NLWs* w, char* ba
   void* p_sl = take(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
   void* p_wr = take(sizeof(double) * 4 * (size_t)(n > 0 ? n : 1));
   void* p_st = take(sizeof(int32_t) * 4096);
+  void* p_un = take(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
   void* p_act = take(sizeof(int32_t) * ((size_t)(s > 0 ? s : 1) + 1));
   if (w) {
     w->geom = (double*)p_geom;
@@ -87,6 +89,7 @@ static size_t ws_layout(int64_t n, int32_t s, int32_t bin_cap, NLWs* w, char* ba
     w->slot = (int32_t*)p_sl;
     w->wr = (double*)p_wr;
     w->scan_tmp = (int32_t*)p_st;
+    w->unsorted = (int32_t*)p_un;
     w->act = (int32_t*)p_act;
   }
   return off;
@@ -332,9 +335,10 @@ __device__ __forceinline__ bool gated_off(const int32_t* __restrict__ gate, int 
 // gate is off cost one load per block.
 
 // ----------------------------------------------------------- active bins
+
 // One workgroup: act[s] = number of bins of the gated-on systems before s,
-// act[n_sys] = their total.  The sort and fill kernels are persistent grids
-// that walk this compact list of active bins.
+// act[n_sys] = their total.  The per-bin fill is a persistent grid that
+// walks this compact list of active bins.
 __global__ __launch_bounds__(1024) void k_active(const int32_t* __restrict__ grid,
                                                  const int32_t* __restrict__ meta, int bin_cap,
                                                  int n_sys, const int32_t* __restrict__ gate,
@@ -469,7 +473,7 @@ __global__ __launch_bounds__(256) void k_scatter(
     const int32_t* __restrict__ sys_start, int n_sys, int B, const int32_t* __restrict__ grid,
     const int32_t* __restrict__ meta, int bin_cap, const int32_t* __restrict__ atom_bin,
     const int32_t* __restrict__ slot, const int32_t* __restrict__ bin_start,
-    int32_t* __restrict__ order, const int32_t* __restrict__ gate) {
+    int32_t* __restrict__ unsorted, const int32_t* __restrict__ gate) {
   const int s = blockIdx.x / B, part = blockIdx.x % B;
   if (s >= n_sys || meta[0] > bin_cap || gated_off(gate, s)) return;
   const int32_t* gr = grid + (size_t)GRID * s;
@@ -478,82 +482,49 @@ __global__ __launch_bounds__(256) void k_scatter(
   for (int i = sys_start[s] + part * blockDim.x + threadIdx.x; i < e; i += B * blockDim.x) {
     // one-bin systems keep atom order; otherwise the atomic slot, sorted later
     const int dst = single ? i : bin_start[atom_bin[i]] + slot[i];
-    order[dst] = i;
+    unsorted[dst] = i;
   }
 }
 
 // ---------------------------------------------------------- sort + wrap
-// Persistent grid of four independent waves per block walking the active-bin
-// list: order each bin's atoms by index (rank sort in the wave's LDS slice;
-// atom indices are unique), reset the bin counter for the next build, and
-// write the wrapped FP64 coordinates in bin order.
-constexpr int SORT_W = 4;
-constexpr int SORT_CAP = 1024;
-
-__device__ __forceinline__ void wave_sync_sort() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
-__global__ __launch_bounds__(64 * SORT_W) void k_sort_wrap(
-    const double* __restrict__ pos, int ps, int n_sys, const double* __restrict__ geom,
-    const int32_t* __restrict__ grid, const int32_t* __restrict__ meta, int bin_cap,
-    const int32_t* __restrict__ act, const int32_t* __restrict__ bin_start,
-    int32_t* __restrict__ bin_count, int32_t* __restrict__ order, double* __restrict__ wr) {
-  __shared__ int32_t keys_all[SORT_W][SORT_CAP];
-  if (meta[0] > bin_cap) return;
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  int32_t* keys = keys_all[wv];
-  const int total = act[n_sys];
-  for (int w = blockIdx.x * SORT_W + wv; w < total; w += gridDim.x * SORT_W) {
-    const int s = active_system(act, n_sys, w);
-    const int32_t* gr = grid + (size_t)GRID * s;
-    const double* g = geom + (size_t)GEOM * s;
-    const int b = gr[6] + (w - act[s]);
-    const int lo = bin_start[b], hi = bin_start[b + 1], n = hi - lo;
-    if (n > 1 && n <= SORT_CAP) {
-      for (int k = lane; k < n; k += 64) keys[k] = order[lo + k];
-      wave_sync_sort();
-      bool uns = false;
-      for (int k = lane; k + 1 < n; k += 64) uns |= keys[k] > keys[k + 1];
-      if (__ballot(uns) != 0ull) {
-        for (int k = lane; k < n; k += 64) {
-          const int v = keys[k];
-          int rank = 0;
-          for (int q = 0; q < n; ++q) rank += keys[q] < v;
-          order[lo + rank] = v;
-        }
-      }
-      __threadfence_block();
-      wave_sync_sort();
-    } else if (n > SORT_CAP) {
-      if (lane == 0) {  // rare: very large bins of multi-bin systems
-        for (int k = lo + 1; k < hi; ++k) {
-          const int v = order[k];
-          int p = k - 1;
-          while (p >= lo && order[p] > v) {
-            order[p + 1] = order[p];
-            --p;
-          }
-          order[p + 1] = v;
-        }
-      }
-      __threadfence_block();
-      wave_sync_sort();
+// One thread per atom of a rebuilt system: its rank among the atoms of its
+// bin (by atom index, so the order is unique and does not depend on the
+// scatter's atomic slots), its place order[bin_start + rank], its wrapped
+// coordinates at that place, and the bin counter reset for the next build.
+// Single-bin systems were scattered in atom order already.
+__global__ __launch_bounds__(256) void k_sort_atom(
+    const double* __restrict__ pos, int ps, const int32_t* __restrict__ sys_start, int n_sys,
+    int B, const double* __restrict__ geom, const int32_t* __restrict__ grid,
+    const int32_t* __restrict__ meta, int bin_cap, const int32_t* __restrict__ atom_bin,
+    const int32_t* __restrict__ bin_start, const int32_t* __restrict__ unsorted,
+    int32_t* __restrict__ bin_count, int32_t* __restrict__ order, double* __restrict__ wr,
+    const int32_t* __restrict__ gate) {
+  const int s = blockIdx.x / B, part = blockIdx.x % B;
+  if (s >= n_sys || meta[0] > bin_cap || gated_off(gate, s)) return;
+  const int32_t* gr = grid + (size_t)GRID * s;
+  const double* g = geom + (size_t)GEOM * s;
+  const bool single = gr[0] * gr[1] * gr[2] == 1;
+  const int e = sys_start[s + 1];
+  for (int i = sys_start[s] + part * blockDim.x + threadIdx.x; i < e; i += B * blockDim.x) {
+    const int b = atom_bin[i];
+    const int lo = bin_start[b], hi = bin_start[b + 1];
+    int dst;
+    if (single) {
+      dst = i;
+    } else {
+      int rank = 0;
+      for (int k = lo; k < hi; ++k) rank += unsorted[k] < i;
+      dst = lo + rank;
     }
-    if (lane == 0) bin_count[b] = 0;
-    for (int k = lo + lane; k < hi; k += 64) {
-      const int i = order[k];
-      const WrapOut o = wrap_atom(pos + (int64_t)ps * i, g, gr[7]);
-      double4 v;
-      v.x = o.wr[0];
-      v.y = o.wr[1];
-      v.z = o.wr[2];
-      v.w = pack_w(o.w[0], o.w[1], o.w[2]);
-      reinterpret_cast<double4*>(wr)[k] = v;
-    }
-    wave_sync_sort();
+    order[dst] = i;
+    const WrapOut o = wrap_atom(pos + (int64_t)ps * i, g, gr[7]);
+    double4 v;
+    v.x = o.wr[0];
+    v.y = o.wr[1];
+    v.z = o.wr[2];
+    v.w = pack_w(o.w[0], o.w[1], o.w[2]);
+    reinterpret_cast<double4*>(wr)[dst] = v;
+    bin_count[b] = 0;
   }
 }
 
@@ -778,12 +749,15 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
 }
 
 // ---------------------------------------------------------- fill, per atom
-// One thread per list row t (atom order[t], in bin order).  The thread walks
-// the stencil images of its bin in the same order as the per-bin fill and the
-// atoms of each image bin in bin order, reading the wrapped coordinates from
-// L2 (neighbouring rows share bins, so the reads coalesce), and writes its
-// row in the order the candidates are met.  No LDS, no staging: sparse bins
-// (few atoms each) cost nothing beyond their candidates.
+// A group of L lanes per list row t (atom order[t], rows in bin order).  The
+// group walks the stencil images of the atom's bin in order; within a
+// candidate range lane `sub` tests candidates kb + sub, kb + sub + L, ...; a
+// ballot over the group compacts the matches, so the row is written in
+// candidate order (the order a single thread would meet them).  Candidates are read from L2:
+// rows of one bin share their candidates, so the reads coalesce.  No LDS and
+// no staging, so sparse bins cost only their candidates, and partial
+// (gated) rebuilds still fill the chip with L lanes per rebuilt atom.
+template <int L>
 __global__ __launch_bounds__(256) void k_fill_atom(
     const int32_t* __restrict__ sys_start, const int32_t* __restrict__ atom_sys,
     int64_t n_atoms, const double* __restrict__ geom, const int32_t* __restrict__ grid,
@@ -792,83 +766,137 @@ __global__ __launch_bounds__(256) void k_fill_atom(
     const double* __restrict__ wr, double rl2, int self_int, int half, int jbits, int tpa,
     int kmax, int64_t ld, uint32_t* __restrict__ nbr, int32_t* __restrict__ count,
     int32_t* __restrict__ status, const int32_t* __restrict__ gate) {
-  const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  const int64_t gid = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  const int64_t t = gid / L;
+  const int sub = (int)(gid % L);
   if (t >= n_atoms || meta[0] > bin_cap) return;
   const int i = order[t];
   const int s = atom_sys[i];
   if (gated_off(gate, s)) return;
+  const int lane = threadIdx.x & 63;
+  const int gbase = lane & ~(L - 1);
+  const uint64_t gmask = (L == 64 ? ~0ull : ((1ull << L) - 1ull)) << gbase;
+  const uint64_t below = ((1ull << sub) - 1ull) << gbase;
   const double* g = geom + (size_t)GEOM * s;
   const int32_t* gr = grid + (size_t)GRID * s;
   const int base = sys_start[s];
   const int off = gr[6];
-  const int wx = 2 * gr[3] + 1, wy = 2 * gr[4] + 1, wz = 2 * gr[5] + 1;
-  const int n_img = wx * wy * wz;
+  const int nbx = gr[0], nby = gr[1], nbz = gr[2];
+  const int mx = gr[3], my = gr[4], mz = gr[5];
   const int local = atom_bin[i] - off;
-  const int bz0 = local % gr[2], by0 = (local / gr[2]) % gr[1], bx0 = local / (gr[1] * gr[2]);
+  const int bz0 = local % nbz, by0 = (local / nbz) % nby, bx0 = local / (nby * nbz);
   const int sbits = (32 - jbits) / 3;
   const int bias = 1 << (sbits - 1);
   const uint32_t fmask = (1u << sbits) - 1u;
   const double4 wi = reinterpret_cast<const double4*>(wr)[t];
   int w_i[3];
   unpack_w(wi.w, w_i);
-  const double g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3], g4 = g[4], g5 = g[5], g6 = g[6],
-               g7 = g[7], g8 = g[8];
   int cnt = 0;
   bool shift_ovf = false;
-  for (int im = 0; im < n_img; ++im) {
-    const int dz = im % wz - gr[5];
-    const int dy = (im / wz) % wy - gr[4];
-    const int dx = im / (wy * wz) - gr[3];
-    const int cx = bx0 + dx, cy = by0 + dy, cz = bz0 + dz;
-    const int sx = floordiv(cx, gr[0]), sy = floordiv(cy, gr[1]), sz = floordiv(cz, gr[2]);
-    const int nbin =
-        off + ((cx - sx * gr[0]) * gr[1] + (cy - sy * gr[1])) * gr[2] + (cz - sz * gr[2]);
-    const int kb = bin_start[nbin], ke = bin_start[nbin + 1];
-    if (kb == ke) continue;
-    // S . L in the reference einsum order (transforms.py:563-565)
-    const double c0 = (double)sx * g0 + (double)sy * g3 + (double)sz * g6;
-    const double c1 = (double)sx * g1 + (double)sy * g4 + (double)sz * g7;
-    const double c2 = (double)sx * g2 + (double)sy * g5 + (double)sz * g8;
-    for (int k = kb; k < ke; ++k) {
-      const double4 wj = reinterpret_cast<const double4*>(wr)[k];
-      const double d2 = dist2_ref(wi.x, wi.y, wi.z, wj.x, wj.y, wj.z, c0, c1, c2);
-      if (!(d2 < rl2)) continue;
-      const int j = order[k];
-      bool match = !(j == i && (sx | sy | sz) == 0 && !self_int);
-      int w_j[3];
-      unpack_w(wj.w, w_j);
-      const int ux = sx + w_i[0] - w_j[0];
-      const int uy = sy + w_i[1] - w_j[1];
-      const int uz = sz + w_i[2] - w_j[2];
-      if (match && half) {
-        if (i != j) {
-          match = i < j;
-        } else {
-          const int first = ux != 0 ? ux : (uy != 0 ? uy : uz);
-          match = first > 0;
+  // Stencil columns (dx, dy); along z the bins of a column are contiguous in
+  // memory, so each run of z-bins with one lattice shift sz is one contiguous
+  // candidate range (image order dx, dy, dz and bin order are kept).
+  for (int dx = -mx; dx <= mx; ++dx) {
+    const int cx = bx0 + dx;
+    const int sx = floordiv(cx, nbx);
+    const int ix = cx - sx * nbx;
+    for (int dy = -my; dy <= my; ++dy) {
+      const int cy = by0 + dy;
+      const int sy = floordiv(cy, nby);
+      const int iy = cy - sy * nby;
+      const int colbase = off + (ix * nby + iy) * nbz;
+      const int cz_end = bz0 + mz;
+      for (int cz = bz0 - mz; cz <= cz_end;) {
+        const int sz = floordiv(cz, nbz);
+        const int iz = cz - sz * nbz;
+        const int run = min(nbz - iz, cz_end - cz + 1);
+        cz += run;
+        const int kb = bin_start[colbase + iz], ke = bin_start[colbase + iz + run];
+        if (kb == ke) continue;
+        // S . L in the reference einsum order (transforms.py:563-565)
+        const double c0 = (double)sx * g[0] + (double)sy * g[3] + (double)sz * g[6];
+        const double c1 = (double)sx * g[1] + (double)sy * g[4] + (double)sz * g[7];
+        const double c2 = (double)sx * g[2] + (double)sy * g[5] + (double)sz * g[8];
+        // two candidates per lane in flight: both loads issue before either test
+        for (int k0 = kb; k0 < ke; k0 += 2 * L) {
+          const int ka = k0 + sub, kc = k0 + L + sub;
+          double4 wa = make_double4(0.0, 0.0, 0.0, 0.0), wc = wa;
+          int ja = 0, jc = 0;
+          if (ka < ke) {
+            wa = reinterpret_cast<const double4*>(wr)[ka];
+            ja = order[ka];
+          }
+          if (kc < ke) {
+            wc = reinterpret_cast<const double4*>(wr)[kc];
+            jc = order[kc];
+          }
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const int k = u == 0 ? ka : kc;
+            const double4 wj = u == 0 ? wa : wc;
+            const int j = u == 0 ? ja : jc;
+            if (k0 + u * L >= ke) break;  // uniform over the group
+            bool match = false;
+            uint32_t word = 0;
+            if (k < ke) {
+              const double d2 = dist2_ref(wi.x, wi.y, wi.z, wj.x, wj.y, wj.z, c0, c1, c2);
+              if (d2 < rl2) {
+                match = !(j == i && (sx | sy | sz) == 0 && !self_int);
+                int w_j[3];
+                unpack_w(wj.w, w_j);
+                const int ux = sx + w_i[0] - w_j[0];
+                const int uy = sy + w_i[1] - w_j[1];
+                const int uz = sz + w_i[2] - w_j[2];
+                if (match && half) {
+                  if (i != j) {
+                    match = i < j;
+                  } else {
+                    const int first = ux != 0 ? ux : (uy != 0 ? uy : uz);
+                    match = first > 0;
+                  }
+                }
+                if (match && (ux < -bias || ux >= bias || uy < -bias || uy >= bias ||
+                              uz < -bias || uz >= bias)) {
+                  shift_ovf = true;
+                  match = false;
+                }
+                word = (uint32_t)(j - base) |
+                       ((((uint32_t)(ux + bias) & fmask) |
+                         (((uint32_t)(uy + bias) & fmask) << sbits) |
+                         (((uint32_t)(uz + bias) & fmask) << (2 * sbits)))
+                        << jbits);
+              }
+            }
+            const uint64_t bal = __ballot(match) & gmask;
+            if (match) {
+              const int pos = cnt + __popcll(bal & below);
+              if (pos < kmax) nbr[ell_index(t, pos, tpa, ld)] = word;
+            }
+            cnt += __popcll(bal);
+          }
         }
       }
-      if (match && (ux < -bias || ux >= bias || uy < -bias || uy >= bias || uz < -bias ||
-                    uz >= bias)) {
-        shift_ovf = true;
-        match = false;
-      }
-      if (!match) continue;
-      if (cnt < kmax)
-        nbr[ell_index(t, cnt, tpa, ld)] =
-            (uint32_t)(j - base) |
-            ((((uint32_t)(ux + bias) & fmask) | (((uint32_t)(uy + bias) & fmask) << sbits) |
-              (((uint32_t)(uz + bias) & fmask) << (2 * sbits)))
-             << jbits);
-      ++cnt;
     }
   }
+  const uint64_t ovf = __ballot(shift_ovf) & gmask;
+  if (sub != 0) return;
   if (cnt > kmax) {
     atomicOr(status, TSX_STATUS_NL_OVERFLOW);
     cnt = kmax;
   }
-  if (shift_ovf) atomicOr(status, TSX_STATUS_SHIFT_OVERFLOW);
+  if (ovf) atomicOr(status, TSX_STATUS_SHIFT_OVERFLOW);
   count[t] = cnt;
+}
+
+template <int L>
+static void launch_fill_atom(hipStream_t st, const int32_t* sys_start, const int32_t* atom_sys,
+                             int64_t n, const NLWs& w, int bin_cap, const int32_t* order,
+                             double rl2, int self_int, int half, int jbits, int tpa, int kmax,
+                             int64_t ld, uint32_t* nbr, int32_t* count, int32_t* status,
+                             const int32_t* gate) {
+  k_fill_atom<L><<<blocks_for(n * L, 256), 256, 0, st>>>(
+      sys_start, atom_sys, n, w.geom, w.grid, w.meta, bin_cap, w.atom_bin, w.bin_start, order,
+      w.wr, rl2, self_int, half, jbits, tpa, kmax, ld, nbr, count, status, gate);
 }
 
 __global__ void k_emit(const uint32_t* __restrict__ nbr, const int32_t* __restrict__ count,
@@ -1004,7 +1032,8 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
                  int32_t bins_per_rlist, int32_t self_interaction, int32_t half, int32_t jbits,
                  int32_t tpa, int32_t kmax, int64_t ld, int32_t bin_cap, int32_t setup, void* ws,
                  uint32_t* nbr, int32_t* count, int32_t* order, double* xref, int32_t* status,
-                 const int32_t* gate, int32_t* counter, double* cell_build, void* stream) {
+                 const int32_t* gate, int32_t* counter, double* cell_build, int32_t fill_mode,
+                 void* stream) {
   TSX_REQUIRE(n_systems > 0, "n_systems must be positive");
   TSX_REQUIRE(n_atoms >= 0 && n_atoms < (1ll << 31), "n_atoms out of range");
   TSX_REQUIRE(pos_stride == 3 || pos_stride == 4, "pos_stride must be 3 or 4");
@@ -1016,6 +1045,7 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
   TSX_REQUIRE(r_list > 0.0, "r_list must be positive");
   TSX_REQUIRE(bins_per_rlist >= 1 && bins_per_rlist <= 4, "bins_per_rlist in [1,4]");
   TSX_REQUIRE(bin_cap >= 1, "bin_cap must be >= 1");
+  TSX_REQUIRE(fill_mode >= TSX_FILL_AUTO && fill_mode <= TSX_FILL_ATOM, "unknown fill_mode");
   hipStream_t st = (hipStream_t)stream;
   NLWs w;
   ws_layout(n_atoms, n_systems, bin_cap, &w, (char*)ws);
@@ -1038,21 +1068,38 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
                                         w.bin_start, gate, counter, cell_col, cell_build);
   TSX_CHECK_LAUNCH();
   k_scatter<<<n_systems * b_atoms, 256, 0, st>>>(sys_start, n_systems, b_atoms, w.grid, w.meta,
-                                                 bin_cap, w.atom_bin, w.slot, w.bin_start, order,
-                                                 gate);
+                                                 bin_cap, w.atom_bin, w.slot, w.bin_start,
+                                                 w.unsorted, gate);
   TSX_CHECK_LAUNCH();
-  // persistent grids over the active-bin list (sizes fixed for graph capture)
-  k_active<<<1, 1024, 0, st>>>(w.grid, w.meta, bin_cap, n_systems, gate, w.act);
+  k_sort_atom<<<n_systems * b_atoms, 256, 0, st>>>(pos, pos_stride, sys_start, n_systems, b_atoms,
+                                                   w.geom, w.grid, w.meta, bin_cap, w.atom_bin,
+                                                   w.bin_start, w.unsorted, w.bin_count, order,
+                                                   w.wr, gate);
   TSX_CHECK_LAUNCH();
-  const int g_sort = blocks_for(bin_cap, SORT_W) < 2048 ? blocks_for(bin_cap, SORT_W) : 2048;
-  k_sort_wrap<<<g_sort, 64 * SORT_W, 0, st>>>(pos, pos_stride, n_systems, w.geom, w.grid,
-                                              w.meta, bin_cap, w.act, w.bin_start, w.bin_count,
-                                              order, w.wr);
-  TSX_CHECK_LAUNCH();
-  k_fill_atom<<<blocks_for(n_atoms, 256), 256, 0, st>>>(
-      sys_start, atom_sys, n_atoms, w.geom, w.grid, w.meta, bin_cap, w.atom_bin, w.bin_start,
-      order, w.wr, r_list * r_list, self_interaction, half, jbits, tpa, kmax, ld, nbr, count,
-      status, gate);
+  const double rl2 = r_list * r_list;
+  // Fill kernel: per-bin LDS staging pays off for dense lists of many
+  // systems, where gated rebuilds touch a few percent of the atoms and one
+  // wave per bin shares the staged candidates among the bin's atoms; the
+  // per-atom kernel wins for sparse bins and for single large systems.
+  bool per_bin = fill_mode == TSX_FILL_BIN;
+  if (fill_mode == TSX_FILL_AUTO) per_bin = bins_per_rlist >= 2 && kmax >= 64 && n_systems > 1;
+  if (per_bin) {
+    k_active<<<1, 1024, 0, st>>>(w.grid, w.meta, bin_cap, n_systems, gate, w.act);
+    TSX_CHECK_LAUNCH();
+    const int g_fill = blocks_for(bin_cap, FILL_W) < 768 ? blocks_for(bin_cap, FILL_W) : 768;
+    k_fill<<<g_fill, 64 * FILL_W, 0, st>>>(sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap,
+                                           w.act, w.bin_start, order, w.wr, rl2,
+                                           self_interaction, half, jbits, tpa, kmax, ld, nbr,
+                                           count, status);
+  } else if (kmax >= 64) {
+    // lanes per atom from the row capacity (dense lists: more candidates per
+    // column); fewer lanes leave more atoms per wave for the column arithmetic
+    launch_fill_atom<4>(st, sys_start, atom_sys, n_atoms, w, bin_cap, order, rl2,
+                        self_interaction, half, jbits, tpa, kmax, ld, nbr, count, status, gate);
+  } else {
+    launch_fill_atom<2>(st, sys_start, atom_sys, n_atoms, w, bin_cap, order, rl2,
+                        self_interaction, half, jbits, tpa, kmax, ld, nbr, count, status, gate);
+  }
   TSX_CHECK_LAUNCH();
   return TSX_OK;
 }

@@ -4,9 +4,10 @@
 // atom, one workgroup per (system, chunk): a single 1M-atom system and 512
 // small ones both fill the chip, and a system's summation order depends only
 // on its own size (a system sums bitwise alike alone or inside any batch).
-// The last workgroup to finish (device-scope counter, fenced) adds each
-// system's chunk partials in chunk order and writes the results.  The
-// counter is returned to zero, so the kernels can be replayed in a graph.
+// When every system fits one chunk, the workgroup owning a system writes its
+// results directly; otherwise a second kernel adds each system's chunk
+// partials in chunk order (the kernel boundary orders the partial writes; a
+// device-scope fence per workgroup would write back the L2).
 //
 // Reference: the per-system index_add sums of pair_potential.py:351-464
 // (energy, stress) and md.py / quantities.py segment sums.
@@ -58,68 +59,69 @@ template <int V>
 __global__ __launch_bounds__(RED_THREADS) void k_seg_reduce(
     const double* __restrict__ e_atom, const double* __restrict__ w_atom,
     const int32_t* __restrict__ sys_start, int chunks, int n_sys, double* __restrict__ part,
-    int32_t* __restrict__ counter, SegOut o, double* __restrict__ stresses) {
+    SegOut o, double* __restrict__ stresses) {
   __shared__ double sh[RED_WAVES][V];
-  __shared__ int last;
-  const int nblk = gridDim.x;
   const int s = blockIdx.x / chunks, c = blockIdx.x % chunks;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int64_t lo = (int64_t)sys_start[s] + (int64_t)c * RED_CH;
   const int64_t end = sys_start[s + 1];
   const int64_t hi = lo + RED_CH < end ? lo + RED_CH : end;
-  if (lo < hi) {
-    double acc[V];
-    for (int q = 0; q < V; ++q) acc[q] = 0.0;
-    for (int64_t k = lo + threadIdx.x; k < hi; k += RED_THREADS) {
-      double v[V];
-      load_vals<V>(e_atom, w_atom, k, v);
-      for (int q = 0; q < V; ++q) acc[q] += v[q];
-    }
-    for (int q = 0; q < V; ++q) {
-      const double t = wave_sum(acc[q]);
-      if (lane == 0) sh[wid][q] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x < V) {
-      double t = 0.0;
-      for (int w = 0; w < RED_WAVES; ++w) t += sh[w][threadIdx.x];
-      part[((int64_t)s * chunks + c) * V + threadIdx.x] = t;
-    }
-    if (V == 7 && stresses) {
-      const double iv = 1.0 / fabs(det33(o.cell_col + 9 * (int64_t)s));
-      for (int64_t k = lo + threadIdx.x; k < hi; k += RED_THREADS) {
-        const double* wk = w_atom + 6 * k;
-        double* st = stresses + 9 * k;
-        st[0] = -wk[0] * iv;
-        st[4] = -wk[1] * iv;
-        st[8] = -wk[2] * iv;
-        st[5] = st[7] = -wk[3] * iv;
-        st[2] = st[6] = -wk[4] * iv;
-        st[1] = st[3] = -wk[5] * iv;
-      }
-    }
-  }
-  // the last workgroup to finish adds the chunk partials of every system
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) last = atomicAdd(counter, 1) == nblk - 1;
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  for (int q = threadIdx.x; q < n_sys; q += RED_THREADS) {
+  double acc[V];
+  for (int q = 0; q < V; ++q) acc[q] = 0.0;
+  for (int64_t k = lo + threadIdx.x; k < hi; k += RED_THREADS) {
     double v[V];
-    for (int k = 0; k < V; ++k) v[k] = 0.0;
-    const int64_t n_q = (int64_t)sys_start[q + 1] - sys_start[q];
-    int nc = (int)((n_q + RED_CH - 1) / RED_CH);
-    nc = nc < chunks ? nc : chunks;  // a too small max_system_atoms truncates, never overreads
-    for (int cc = 0; cc < nc; ++cc)
-      for (int k = 0; k < V; ++k) v[k] += part[((int64_t)q * chunks + cc) * V + k];
-    finish_system<V>(o, q, v);
+    load_vals<V>(e_atom, w_atom, k, v);
+    for (int q = 0; q < V; ++q) acc[q] += v[q];
   }
+  for (int q = 0; q < V; ++q) {
+    const double t = wave_sum(acc[q]);
+    if (lane == 0) sh[wid][q] = t;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    if (o.advance && o.step) o.step[0] += 1;
-    counter[0] = 0;
+    double v[V];
+    for (int q = 0; q < V; ++q) {
+      double t = 0.0;
+      for (int w = 0; w < RED_WAVES; ++w) t += sh[w][q];
+      v[q] = t;
+    }
+    if (chunks == 1) {
+      finish_system<V>(o, s, v);
+      if (o.advance && o.step && s == 0) o.step[0] += 1;
+    } else {
+      for (int q = 0; q < V; ++q) part[((int64_t)s * chunks + c) * V + q] = v[q];
+    }
   }
+  if (V == 7 && stresses && lo < hi) {
+    const double iv = 1.0 / fabs(det33(o.cell_col + 9 * (int64_t)s));
+    for (int64_t k = lo + threadIdx.x; k < hi; k += RED_THREADS) {
+      const double* wk = w_atom + 6 * k;
+      double* st = stresses + 9 * k;
+      st[0] = -wk[0] * iv;
+      st[4] = -wk[1] * iv;
+      st[8] = -wk[2] * iv;
+      st[5] = st[7] = -wk[3] * iv;
+      st[2] = st[6] = -wk[4] * iv;
+      st[1] = st[3] = -wk[5] * iv;
+    }
+  }
+}
+
+// systems larger than one chunk: add the chunk partials in chunk order
+template <int V>
+__global__ void k_seg_finish(const int32_t* __restrict__ sys_start, int chunks, int n_sys,
+                             const double* __restrict__ part, SegOut o) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_sys) return;
+  double v[V];
+  for (int k = 0; k < V; ++k) v[k] = 0.0;
+  const int64_t n_s = (int64_t)sys_start[s + 1] - sys_start[s];
+  int nc = (int)((n_s + RED_CH - 1) / RED_CH);
+  nc = nc < chunks ? nc : chunks;  // a too small max_system_atoms truncates, never overreads
+  for (int c = 0; c < nc; ++c)
+    for (int k = 0; k < V; ++k) v[k] += part[((int64_t)s * chunks + c) * V + k];
+  finish_system<V>(o, s, v);
+  if (o.advance && o.step && s == 0) o.step[0] += 1;
 }
 
 }  // namespace tsx
@@ -143,9 +145,8 @@ static int seg_launch(int v, const double* e_atom, const double* w_atom, const d
                       int32_t advance, int32_t* gate_clear, void* ws, hipStream_t st) {
   TSX_REQUIRE(n_systems > 0, "n_systems must be positive");
   TSX_REQUIRE(max_system_atoms >= 0, "max_system_atoms must be >= 0");
-  TSX_REQUIRE(ws != nullptr, "workspace of tsx_reduce_ws_bytes required (zeroed once)");
+  TSX_REQUIRE(ws != nullptr, "workspace of tsx_reduce_ws_bytes required");
   TSX_REQUIRE(!advance || step, "step counter required to advance");
-  int32_t* counter = (int32_t*)ws;
   double* part = (double*)((char*)ws + 64);
   SegOut o{energy, stress, cell_col, gate_clear, step, advance};
   const int chunks = chunks_for(max_system_atoms);
@@ -153,11 +154,20 @@ static int seg_launch(int v, const double* e_atom, const double* w_atom, const d
   TSX_REQUIRE(nblk < (1ll << 31), "too many reduction blocks");
   if (v == 1)
     k_seg_reduce<1><<<(int)nblk, RED_THREADS, 0, st>>>(e_atom, nullptr, sys_start, chunks,
-                                                        n_systems, part, counter, o, nullptr);
+                                                        n_systems, part, o, nullptr);
   else
     k_seg_reduce<7><<<(int)nblk, RED_THREADS, 0, st>>>(e_atom, w_atom, sys_start, chunks,
-                                                        n_systems, part, counter, o, stresses);
+                                                        n_systems, part, o, stresses);
   TSX_CHECK_LAUNCH();
+  if (chunks > 1) {
+    if (v == 1)
+      k_seg_finish<1><<<blocks_for(n_systems, 64), 64, 0, st>>>(sys_start, chunks, n_systems,
+                                                                part, o);
+    else
+      k_seg_finish<7><<<blocks_for(n_systems, 64), 64, 0, st>>>(sys_start, chunks, n_systems,
+                                                                part, o);
+    TSX_CHECK_LAUNCH();
+  }
   return TSX_OK;
 }
 

@@ -144,8 +144,10 @@ class NeighborList:
         self_interaction: bool = False,
         tpa: int = 4,
         bins_per_rlist: int = 1,
+        fill_mode: int = 0,
     ):
         self.r_list = float(r_list)
+        self.fill_mode = int(fill_mode)  # TSX_FILL_AUTO / _BIN / _ATOM
         self.half = bool(half)
         self.self_interaction = bool(self_interaction)
         self.tpa = int(tpa)
@@ -217,7 +219,7 @@ class NeighborList:
             self.r_list, self.bins_per_rlist, int(self.self_interaction), int(self.half),
             self.jbits, self.tpa, self.kmax, self.ld, self.bin_cap, int(setup), ptr(self.ws),
             ptr(self.nbr), ptr(self.count), ptr(self.order), ptr(xref), ptr(self.status),
-            ptr(gate), ptr(counter), ptr(cell_build), stream(),
+            ptr(gate), ptr(counter), ptr(cell_build), self.fill_mode, stream(),
         )
 
     def build(
