@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import platform
 import sys
@@ -36,12 +37,21 @@ MORSE = dict(sigma=2.866, epsilon=0.3429, alpha=1.3588)
 # HBM bytes per dispatch of the fused-step kernel, from two rocprofv3 --pmc runs of this
 # script (FETCH_SIZE, WRITE_SIZE in KiB per dispatch, averaged over the dispatches), each
 # scaled by what the same counter reads for a 256 MiB device copy (FETCH_SIZE x 2.000,
-# WRITE_SIZE x 1.000).  Keyed by (n_systems, skin, tpa); the summary with its commands is
-# profiles/k_md_fused__cu_morse_nvt_512sys.md.
+# WRITE_SIZE x 1.000).  Keyed by (workload, skin, tpa); the summaries with their commands
+# are profiles/k_md_fused__cu_morse_nvt_512sys.md and profiles/k_md_fused__lj_fluid_1m_nve.md.
 MEASURED_TRAFFIC = {
-    (512, 0.72, 2): {"fetch_kib": 107098.6, "fetch_factor": 2.000, "write_kib": 36122.8,
-                     "write_factor": 1.000},
+    ("cu_morse_nvt", 0.72, 2): {"fetch_kib": 107098.9, "fetch_factor": 2.000,
+                                "write_kib": 35924.8, "write_factor": 1.000},
+    ("lj1m_nve", 0.3, 4): {"fetch_kib": 307970.8, "fetch_factor": 2.000,
+                           "write_kib": 112049.8, "write_factor": 1.000},
 }
+
+
+def measured_traffic(workload: str, skin: float, tpa: int):
+    meas = MEASURED_TRAFFIC.get((workload, round(skin, 3), tpa))
+    if meas is None:
+        return None
+    return 1024.0 * (meas["fetch_kib"] * meas["fetch_factor"] + meas["write_kib"] * meas["write_factor"])
 
 
 def log(msg: str) -> None:
@@ -93,6 +103,110 @@ def plain_gpu(state, model, steps, warmup):
         md = tsa.nvt_langevin_step(md, model, dt=dt, kT=kT, gamma=gamma)
     torch.cuda.synchronize()
     return st.n_atoms * steps / (time.perf_counter() - t0)
+
+
+def _torch_pairs(pos, cell_col, system_idx, n_sys, rc):
+    """Full neighbor list (i, j, S) with |r_j - r_i + S.L| < rc by torch ops:
+    the reference's linked-cell idea (torch_nl.py:156-215, transforms.py:
+    843-1006) restated for a GPU batch: wrap, bin into cells of edge >= rc,
+    sort by (system, bin), expand the 27-bin stencil of every atom into
+    candidates, keep those within rc.  Needs every cell width >= rc."""
+    dev = pos.device
+    row = cell_col.transpose(1, 2)  # row vectors a_k
+    inv = torch.linalg.inv(row)
+    frac = torch.bmm(pos.unsqueeze(1), inv[system_idx]).squeeze(1)
+    wrap = torch.floor(frac)
+    frac = frac - wrap
+    width = 1.0 / torch.linalg.norm(inv, dim=1)  # plane spacings [S,3]
+    nb = torch.clamp(torch.floor(width / rc), min=1).to(torch.int64)  # [S,3]
+    ib = torch.clamp((frac * nb[system_idx]).to(torch.int64), max=nb[system_idx] - 1)
+    nbin_s = nb.prod(1)
+    off = torch.cumsum(nbin_s, 0) - nbin_s
+    nbn = nb[system_idx]
+    key = off[system_idx] + (ib[:, 0] * nbn[:, 1] + ib[:, 1]) * nbn[:, 2] + ib[:, 2]
+    order = torch.argsort(key, stable=True)
+    skey = key[order]
+    total_bins = int(nbin_s.sum())
+    bins = torch.arange(total_bins + 1, device=dev)
+    bstart = torch.searchsorted(skey, bins)
+    d = torch.tensor([-1, 0, 1], device=dev)
+    sten = torch.stack(torch.meshgrid(d, d, d, indexing="ij"), -1).reshape(27, 3)
+    out_i, out_j, out_s = [], [], []
+    n = pos.shape[0]
+    for a0 in range(0, n, 32768):  # atom chunks bound the candidate tensors
+        a1 = min(n, a0 + 32768)
+        ia = torch.arange(a0, a1, device=dev)
+        sa = system_idx[a0:a1]
+        cb = ib[a0:a1].unsqueeze(1) + sten.unsqueeze(0)  # [n,27,3]
+        nbs = nb[sa].unsqueeze(1)
+        shift = torch.div(cb, nbs, rounding_mode="floor")
+        cbw = cb - shift * nbs
+        nkey = off[sa].unsqueeze(1) + (cbw[..., 0] * nbs[..., 1] + cbw[..., 1]) * nbs[..., 2] + cbw[..., 2]
+        lo, hi = bstart[nkey], bstart[nkey + 1]
+        cnt = (hi - lo).reshape(-1)
+        src = torch.repeat_interleave(torch.arange(cnt.numel(), device=dev), cnt)
+        first = torch.cumsum(cnt, 0) - cnt
+        k = lo.reshape(-1)[src] + (torch.arange(src.numel(), device=dev) - first[src])
+        i = ia[torch.div(src, 27, rounding_mode="floor")]
+        j = order[k]
+        s_img = shift.reshape(-1, 3)[src].to(pos.dtype) + wrap[i] - wrap[j]  # unwrapped shifts
+        si = system_idx[i]
+        sl = (s_img[:, 0:1] * row[si, 0] + s_img[:, 1:2] * row[si, 1] + s_img[:, 2:3] * row[si, 2])
+        dr = pos[j] - pos[i] + sl
+        keep = ((dr * dr).sum(1) < rc * rc) & ~((i == j) & (s_img == 0).all(1))
+        out_i.append(i[keep])
+        out_j.append(j[keep])
+        out_s.append(s_img[keep])
+    return torch.cat(out_i), torch.cat(out_j), torch.cat(out_s)
+
+
+def plain_torch(state, steps, warmup):
+    """The reference pipeline restated in PyTorch ops on the GPU (a torch
+    linked-cell list rebuilt every step, autograd forces of the Morse pair
+    energy, BAOAB in torch ops; no HIP kernels, no graph capture)."""
+    kT = 1000.0 * tsa.MetalUnits.temperature
+    dt = 0.002 * tsa.MetalUnits.time
+    gamma = 10.0 / tsa.MetalUnits.time
+    dev = state.device
+    x = state.positions.clone()
+    m = state.masses.unsqueeze(-1)
+    idx, cell, n_sys = state.system_idx, state.cell, state.n_systems
+    D, a, r0 = MORSE["epsilon"], MORSE["alpha"], MORSE["sigma"]
+    g = torch.Generator(device=dev).manual_seed(7)
+
+    def forces(xx):
+        i, j, s_img = _torch_pairs(xx.detach(), cell, idx, n_sys, 6.0)
+        xr = xx.detach().requires_grad_(True)
+        row = cell.transpose(1, 2)
+        si = idx[i]
+        dr = xr[j] - xr[i] + (s_img[:, 0:1] * row[si, 0] + s_img[:, 1:2] * row[si, 1]
+                              + s_img[:, 2:3] * row[si, 2])
+        r = dr.norm(dim=1)
+        e = 0.5 * (D * (1.0 - torch.exp(-a * (r - r0))) ** 2 - D).sum()
+        (f,) = torch.autograd.grad(e, xr)
+        return -f
+
+    p = torch.randn(x.shape, generator=g, device=dev, dtype=x.dtype) * torch.sqrt(m * kT)
+    f = forces(x)
+    c1 = math.exp(-gamma * dt)
+    c2 = math.sqrt(kT * (1 - c1 * c1))
+
+    def step(x, p, f):
+        p = p + 0.5 * dt * f
+        x = x + 0.5 * dt * p / m
+        p = c1 * p + c2 * torch.sqrt(m) * torch.randn(x.shape, generator=g, device=dev, dtype=x.dtype)
+        x = x + 0.5 * dt * p / m
+        f = forces(x)
+        return x, p + 0.5 * dt * f, f
+
+    for _ in range(warmup):
+        x, p, f = step(x, p, f)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        x, p, f = step(x, p, f)
+    torch.cuda.synchronize()
+    return state.n_atoms * steps / (time.perf_counter() - t0)
 
 
 def cpu_baseline(n_sys=12, steps=2, seed=0):
@@ -179,15 +293,14 @@ def bench_cu_morse(args) -> dict:
     achieved = bytes_launch / (ms_kernel * 1e-3) / 1e9
     log(f"fused kernel: {ms_kernel * 1e3:.1f} us, k_list {k_mean:.1f}, {achieved:.0f} GB/s")
 
-    traffic = None
-    meas = MEASURED_TRAFFIC.get((args.n_systems, round(skin, 3), tpa))
-    if meas is not None:
-        traffic = 1024.0 * (meas["fetch_kib"] * meas["fetch_factor"] + meas["write_kib"] * meas["write_factor"])
+    traffic = measured_traffic("cu_morse_nvt", skin, tpa) if args.n_systems == 512 else None
 
-    plain = None
+    plain = plain_hip = None
     if not args.no_plain:
-        plain = plain_gpu(state, model, args.plain_steps, 1)
-        log(f"plain per-step GPU path: {plain:.4e} atom-steps/s")
+        plain = plain_torch(state, args.plain_steps, 1)
+        log(f"plain per-step GPU path (PyTorch ops): {plain:.4e} atom-steps/s")
+        plain_hip = plain_gpu(state, model, args.plain_steps, 1)
+        log(f"per-step HIP kernels, no graph, list every step: {plain_hip:.4e} atom-steps/s")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_systems)
@@ -220,10 +333,16 @@ def bench_cu_morse(args) -> dict:
             "list_rebuilds_in_timed_steps": n_reb,
             "mean_list_neighbors": round(k_mean, 2),
         },
-        "plain_gpu": None if plain is None else {"value": plain, "unit": "atom-steps/s",
-                                                  "steps": args.plain_steps,
-                                                  "path": "nvt_langevin_step + model(state), list rebuilt every step"},
+        "plain_gpu": None if plain is None else {
+            "value": plain, "unit": "atom-steps/s", "steps": args.plain_steps,
+            "path": "reference pipeline in PyTorch ops on the GPU: torch linked-cell list every "
+                    "step, autograd Morse forces, BAOAB in torch ops, no graph"},
         "batched_vs_plain": None if plain is None else value / plain,
+        "per_step_hip": None if plain_hip is None else {
+            "value": plain_hip, "unit": "atom-steps/s", "steps": args.plain_steps,
+            "path": "nvt_langevin_step + model(state): HIP kernels per step, list rebuilt every "
+                    "step, no graph capture"},
+        "batched_vs_per_step_hip": None if plain_hip is None else value / plain_hip,
         "roofline": {
             "bound": "hbm",
             "kernel": "k_md_fused (tsx_md_fused_step)",
@@ -284,7 +403,8 @@ def bench_lj1m(args) -> dict:
          "mean_list_neighbors": round(eng.mean_neighbors, 2)},
         roofline={"bound": "hbm", "kernel": "k_md_fused (tsx_md_fused_step)", "achieved": achieved,
                   "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                  "bytes_per_launch": bytes_launch, "kernel_ms": ms_kernel, "traffic": None},
+                  "bytes_per_launch": bytes_launch, "kernel_ms": ms_kernel,
+                  "traffic": measured_traffic("lj1m_nve", args.skin or 0.3, tpa)},
     )
 
 

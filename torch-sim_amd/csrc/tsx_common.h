@@ -49,6 +49,22 @@ __device__ __forceinline__ int xcd_block() {
   return x * q + (x < r ? x : r) + (b >> 3);
 }
 
+// ------------------------------------------------------ neighbour words
+// One 32-bit word per list entry: the neighbour's index within its system in
+// the low jbits, then three sbits-wide two's-complement image-shift fields
+// (S'_x, S'_y, S'_z), sbits = (32 - jbits) / 3.  A zero shift is a zero high
+// part; each field decodes with one signed bitfield extract.
+__device__ __forceinline__ uint32_t nl_word(int j_local, int sx, int sy, int sz, int jbits,
+                                            int sbits) {
+  const uint32_t m = (1u << sbits) - 1u;
+  return (uint32_t)j_local | ((((uint32_t)sx & m) | (((uint32_t)sy & m) << sbits) |
+                               (((uint32_t)sz & m) << (2 * sbits)))
+                              << jbits);
+}
+__device__ __forceinline__ int nl_shift(uint32_t w, int jbits, int sbits, int k) {
+  return __builtin_amdgcn_sbfe((int)w, (unsigned)(jbits + k * sbits), (unsigned)sbits);
+}
+
 // ------------------------------------------------------ wave64 reductions
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

@@ -138,17 +138,14 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_pair_grad(
       wiz = w[3 * (int64_t)i + 2];
     }
     const int sbits = (32 - jbits) / 3;
-    const int bias = 1 << (sbits - 1);
-    const uint32_t fmask = (1u << sbits) - 1u;
     const uint32_t jmask = (1u << jbits) - 1u;
     const int cnt = count[t];
     for (int n = lane; n < cnt; n += TPA) {
       const uint32_t word = nbr[ell_idx(t, n, TPA, ld)];
       const int j = base + (int)(word & jmask);
-      const uint32_t sf = word >> jbits;
-      const int sx = (int)(sf & fmask) - bias;
-      const int sy = (int)((sf >> sbits) & fmask) - bias;
-      const int sz = (int)((sf >> (2 * sbits)) & fmask) - bias;
+      const int sx = nl_shift(word, jbits, sbits, 0);
+      const int sy = nl_shift(word, jbits, sbits, 1);
+      const int sz = nl_shift(word, jbits, sbits, 2);
       const double* pj = pos + PS * (int64_t)j;
       double dx = pj[0] - xi, dy = pj[1] - yi, dz = pj[2] - zi;
       const bool shifted = (sx | sy | sz) != 0;

@@ -596,7 +596,6 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
   int32_t* ipre = ipre_all[wv];
   const int sbits = (32 - jbits) / 3;
   const int bias = 1 << (sbits - 1);
-  const uint32_t fmask = (1u << sbits) - 1u;
   const int total = act[n_sys];
   for (int wi_ = blockIdx.x * FILL_W + wv; wi_ < total; wi_ += gridDim.x * FILL_W) {
     const int s = active_system(act, n_sys, wi_);
@@ -719,10 +718,7 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
                   shift_ovf = true;
                   match = false;
                 }
-                word = (uint32_t)(j - base) |
-                       ((((uint32_t)(ux + bias) & fmask) | (((uint32_t)(uy + bias) & fmask) << sbits) |
-                         (((uint32_t)(uz + bias) & fmask) << (2 * sbits)))
-                        << jbits);
+                word = nl_word(j - base, ux, uy, uz, jbits, sbits);
               }
             }
             const uint64_t bal = __ballot(match);
@@ -787,7 +783,6 @@ __global__ __launch_bounds__(256) void k_fill_atom(
   const int bz0 = local % nbz, by0 = (local / nbz) % nby, bx0 = local / (nby * nbz);
   const int sbits = (32 - jbits) / 3;
   const int bias = 1 << (sbits - 1);
-  const uint32_t fmask = (1u << sbits) - 1u;
   const double4 wi = reinterpret_cast<const double4*>(wr)[t];
   int w_i[3];
   unpack_w(wi.w, w_i);
@@ -860,11 +855,7 @@ __global__ __launch_bounds__(256) void k_fill_atom(
                   shift_ovf = true;
                   match = false;
                 }
-                word = (uint32_t)(j - base) |
-                       ((((uint32_t)(ux + bias) & fmask) |
-                         (((uint32_t)(uy + bias) & fmask) << sbits) |
-                         (((uint32_t)(uz + bias) & fmask) << (2 * sbits)))
-                        << jbits);
+                word = nl_word(j - base, ux, uy, uz, jbits, sbits);
               }
             }
             const uint64_t bal = __ballot(match) & gmask;
@@ -910,20 +901,17 @@ __global__ void k_emit(const uint32_t* __restrict__ nbr, const int32_t* __restri
   const int s = atom_sys[i];
   const int base = sys_start[s];
   const int sbits = (32 - jbits) / 3;
-  const int bias = 1 << (sbits - 1);
-  const uint32_t fmask = (1u << sbits) - 1u;
   const uint32_t jmask = (jbits >= 32) ? 0xffffffffu : ((1u << jbits) - 1u);
   int64_t p = row_ptr[t];
   const int c = count[t];
   for (int n = 0; n < c; ++n, ++p) {
     const uint32_t w = nbr[ell_index(t, n, tpa, ld)];
-    const uint32_t sf = w >> jbits;
     map[p] = i;
     map[n_pairs + p] = base + (int)(w & jmask);
     sysmap[p] = s;
-    shifts[3 * p + 0] = (double)((int)(sf & fmask) - bias);
-    shifts[3 * p + 1] = (double)((int)((sf >> sbits) & fmask) - bias);
-    shifts[3 * p + 2] = (double)((int)((sf >> (2 * sbits)) & fmask) - bias);
+    shifts[3 * p + 0] = (double)nl_shift(w, jbits, sbits, 0);
+    shifts[3 * p + 1] = (double)nl_shift(w, jbits, sbits, 1);
+    shifts[3 * p + 2] = (double)nl_shift(w, jbits, sbits, 2);
   }
 }
 

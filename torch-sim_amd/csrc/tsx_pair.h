@@ -43,6 +43,7 @@ __host__ __device__ inline PairP make_pair_params(int kind, double sigma, double
 // design, and r^2 > 0 is never denormal here.
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
 template <int KIND>
 __device__ __forceinline__ void pair_eval(float r2, const PairP& p, float& e, float& fs) {
@@ -59,20 +60,22 @@ __device__ __forceinline__ void pair_eval(float r2, const PairP& p, float& e, fl
     }
   } else if (KIND == TSX_MORSE) {
     if (r2 > 0.f) {
-      const float r = fsqrt(r2);
+      // one reciprocal square root gives r = r2 / sqrt(r2) and 1 / r
+      const float ir = frsq(r2);
+      const float r = r2 * ir;
       const float x = __expf(-p.c * (r - p.a));
       const float omx = 1.f - x;
       e = p.b * omx * omx - p.b;
-      fs = -p.d * x * omx * frcp(r);
+      fs = -p.d * x * omx * ir;
     } else {
       e = 0.f;
       fs = 0.f;
     }
   } else {
-    const float r = fsqrt(r2);
+    const float ir = r2 > 0.f ? frsq(r2) : 0.f;
+    const float r = r2 * ir;
     if (r < p.a) {
       const float t = 1.f - r * p.e;
-      const float ir = r > 0.f ? frcp(r) : 0.f;
       if (p.c == 2.f) {
         e = 0.5f * p.b * t * t;
         fs = p.d * t * ir;
@@ -140,8 +143,8 @@ struct RowAcc {
 // requested before any is used, so each lane keeps UNR gathers in flight.
 constexpr int UNR = 4;
 
-template <int KIND, int TPA, bool VIRIAL, int PS>
-__device__ __forceinline__ RowAcc accumulate_row(
+template <int KIND, int TPA, bool VIRIAL, int PS, bool MULTI>
+__device__ __forceinline__ RowAcc accumulate_row_impl(
     const PairP* sp, int ns, const int32_t* __restrict__ species,
     const double* __restrict__ pos, const double* C, int base, double xi, double yi, double zi,
     int ti, const uint32_t* __restrict__ nbr, int64_t t, int cnt, int lane, int jbits,
@@ -153,9 +156,11 @@ __device__ __forceinline__ RowAcc accumulate_row(
   const double L10 = C[1], L11 = C[4], L12 = C[7];
   const double L20 = C[2], L21 = C[5], L22 = C[8];
   const int sbits = (32 - jbits) / 3;
-  const int bias = 1 << (sbits - 1);
-  const uint32_t fmask = (1u << sbits) - 1u;
   const uint32_t jmask = (1u << jbits) - 1u;
+  // the row's system starts at pbase: a neighbour is pbase + PS * j_local
+  const double* __restrict__ pbase = pos + (int64_t)PS * base;
+  const int32_t* __restrict__ sbase = species ? species + base : nullptr;
+  const PairP p0 = sp[MULTI ? 0 : ti * ns];  // single species: parameters in registers
   // entry n = lane + TPA * k of this row lives at wp[k * stride]
   const uint32_t* wp = nbr + t * TPA + lane;
   const int64_t stride = ld * TPA;
@@ -168,25 +173,24 @@ __device__ __forceinline__ RowAcc accumulate_row(
     int tj[UNR];
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
-      const double* pj = pos + PS * (int64_t)(base + (int)(w[u] & jmask));
+      const uint32_t jl = w[u] & jmask;
+      const double* pj = pbase + PS * jl;
       px[u] = pj[0];
       py[u] = pj[1];
       pz[u] = pj[2];
       tj[u] = 0;
-      if (ns > 1) tj[u] = species ? species[base + (int)(w[u] & jmask)] : (int)__double_as_longlong(pj[3]);
+      if (MULTI) tj[u] = sbase ? sbase[jl] : (int)__double_as_longlong(pj[3]);
     }
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
       if (k0 + u >= iters) break;
-      const uint32_t sf = w[u] >> jbits;
       double dx = px[u] - xi;
       double dy = py[u] - yi;
       double dz = pz[u] - zi;
-      if (sf != (((uint32_t)bias) | ((uint32_t)bias << sbits) |
-                 ((uint32_t)bias << (2 * sbits)))) {  // S' != 0
-        const double fxs = (double)((int)(sf & fmask) - bias);
-        const double fys = (double)((int)((sf >> sbits) & fmask) - bias);
-        const double fzs = (double)((int)((sf >> (2 * sbits)) & fmask) - bias);
+      if ((w[u] >> jbits) != 0u) {  // S' != 0
+        const double fxs = (double)nl_shift(w[u], jbits, sbits, 0);
+        const double fys = (double)nl_shift(w[u], jbits, sbits, 1);
+        const double fzs = (double)nl_shift(w[u], jbits, sbits, 2);
         dx += fxs * L00 + fys * L10 + fzs * L20;
         dy += fxs * L01 + fys * L11 + fzs * L21;
         dz += fxs * L02 + fys * L12 + fzs * L22;
@@ -194,7 +198,7 @@ __device__ __forceinline__ RowAcc accumulate_row(
       const double d2 = dx * dx + dy * dy + dz * dz;
       if (!(d2 < rc2)) continue;
       float e, fs;
-      pair_eval<KIND>((float)d2, sp[ti * ns + tj[u]], e, fs);
+      pair_eval<KIND>((float)d2, MULTI ? sp[ti * ns + tj[u]] : p0, e, fs);
       const double f = (double)fs;
       a.fx = fma(-f, dx, a.fx);
       a.fy = fma(-f, dy, a.fy);
@@ -221,6 +225,24 @@ __device__ __forceinline__ RowAcc accumulate_row(
       for (int c = 0; c < 6; ++c) a.w[c] += __shfl_xor(a.w[c], off, 64);
   }
   return a;
+}
+
+// Row walk dispatched on the number of species: one species runs a loop with
+// its parameters held in registers, several species look the pair up per
+// neighbour.
+template <int KIND, int TPA, bool VIRIAL, int PS>
+__device__ __forceinline__ RowAcc accumulate_row(
+    const PairP* sp, int ns, const int32_t* __restrict__ species,
+    const double* __restrict__ pos, const double* C, int base, double xi, double yi, double zi,
+    int ti, const uint32_t* __restrict__ nbr, int64_t t, int cnt, int lane, int jbits,
+    int64_t ld, double rc2) {
+  if (ns > 1)
+    return accumulate_row_impl<KIND, TPA, VIRIAL, PS, true>(sp, ns, species, pos, C, base, xi,
+                                                            yi, zi, ti, nbr, t, cnt, lane, jbits,
+                                                            ld, rc2);
+  return accumulate_row_impl<KIND, TPA, VIRIAL, PS, false>(sp, ns, species, pos, C, base, xi, yi,
+                                                           zi, ti, nbr, t, cnt, lane, jbits, ld,
+                                                           rc2);
 }
 
 

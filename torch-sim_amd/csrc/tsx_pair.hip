@@ -66,17 +66,14 @@ __global__ __launch_bounds__(256) void k_pair(
                  zi = pos[PS * (int64_t)i + 2];
     const int ti = ns > 1 ? species[i] : 0;
     const int sbits = (32 - jbits) / 3;
-    const int bias = 1 << (sbits - 1);
-    const uint32_t fmask = (1u << sbits) - 1u;
     const uint32_t jmask = (1u << jbits) - 1u;
     const int cnt = count[t];
     for (int n = lane; n < cnt; n += TPA) {
       const uint32_t w = nbr[ell_idx(t, n, TPA, ld)];
       const int j = base + (int)(w & jmask);
-      const uint32_t sf = w >> jbits;
-      const int sx = (int)(sf & fmask) - bias;
-      const int sy = (int)((sf >> sbits) & fmask) - bias;
-      const int sz = (int)((sf >> (2 * sbits)) & fmask) - bias;
+      const int sx = nl_shift(w, jbits, sbits, 0);
+      const int sy = nl_shift(w, jbits, sbits, 1);
+      const int sz = nl_shift(w, jbits, sbits, 2);
       double dx = pos[PS * (int64_t)j] - xi;
       double dy = pos[PS * (int64_t)j + 1] - yi;
       double dz = pos[PS * (int64_t)j + 2] - zi;

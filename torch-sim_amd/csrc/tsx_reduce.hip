@@ -107,21 +107,38 @@ __global__ __launch_bounds__(RED_THREADS) void k_seg_reduce(
   }
 }
 
-// systems larger than one chunk: add the chunk partials in chunk order
+// systems larger than one chunk: one workgroup per system adds its chunk
+// partials (strided per thread, then the fixed-order block tree)
 template <int V>
-__global__ void k_seg_finish(const int32_t* __restrict__ sys_start, int chunks, int n_sys,
-                             const double* __restrict__ part, SegOut o) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n_sys) return;
-  double v[V];
-  for (int k = 0; k < V; ++k) v[k] = 0.0;
+__global__ __launch_bounds__(RED_THREADS) void k_seg_finish(const int32_t* __restrict__ sys_start,
+                                                            int chunks, int n_sys,
+                                                            const double* __restrict__ part,
+                                                            SegOut o) {
+  __shared__ double sh[RED_WAVES][V];
+  const int s = blockIdx.x;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int64_t n_s = (int64_t)sys_start[s + 1] - sys_start[s];
   int nc = (int)((n_s + RED_CH - 1) / RED_CH);
   nc = nc < chunks ? nc : chunks;  // a too small max_system_atoms truncates, never overreads
-  for (int c = 0; c < nc; ++c)
-    for (int k = 0; k < V; ++k) v[k] += part[((int64_t)s * chunks + c) * V + k];
-  finish_system<V>(o, s, v);
-  if (o.advance && o.step && s == 0) o.step[0] += 1;
+  double acc[V];
+  for (int k = 0; k < V; ++k) acc[k] = 0.0;
+  for (int c = threadIdx.x; c < nc; c += RED_THREADS)
+    for (int k = 0; k < V; ++k) acc[k] += part[((int64_t)s * chunks + c) * V + k];
+  for (int k = 0; k < V; ++k) {
+    const double t = wave_sum(acc[k]);
+    if (lane == 0) sh[wid][k] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v[V];
+    for (int k = 0; k < V; ++k) {
+      double t = 0.0;
+      for (int w = 0; w < RED_WAVES; ++w) t += sh[w][k];
+      v[k] = t;
+    }
+    finish_system<V>(o, s, v);
+    if (o.advance && o.step && s == 0) o.step[0] += 1;
+  }
 }
 
 }  // namespace tsx
@@ -161,11 +178,9 @@ static int seg_launch(int v, const double* e_atom, const double* w_atom, const d
   TSX_CHECK_LAUNCH();
   if (chunks > 1) {
     if (v == 1)
-      k_seg_finish<1><<<blocks_for(n_systems, 64), 64, 0, st>>>(sys_start, chunks, n_systems,
-                                                                part, o);
+      k_seg_finish<1><<<n_systems, RED_THREADS, 0, st>>>(sys_start, chunks, n_systems, part, o);
     else
-      k_seg_finish<7><<<blocks_for(n_systems, 64), 64, 0, st>>>(sys_start, chunks, n_systems,
-                                                                part, o);
+      k_seg_finish<7><<<n_systems, RED_THREADS, 0, st>>>(sys_start, chunks, n_systems, part, o);
     TSX_CHECK_LAUNCH();
   }
   return TSX_OK;
