@@ -42,7 +42,17 @@ PINNING = {
     "stress=-(1/V)dE/deps by finite differences; half==full; sum F=0",
     "md.*": "BAOAB one-step closed form with supplied noise; NVE energy drift; "
     "COM removal; batch==single",
-    "fire.*": "P<0 branch closed form (test_optimizers.py:600-662); convergence of "
-    "perfect lattices; Frechet adjoint identity vs 9-direction sum",
+    "fire.ase_fire_step": "P<0 branch (test_optimizers.py:600-662); mixing known "
+    "answers with per-system norms; parallel-velocity identity; dt_max and max_step "
+    "clamps; first step uses raw forces; Frechet first step = scalar exp scaling",
+    "fire.vv_fire_step": "P<0 zeroes velocities; n_pos ordering vs ASE flavour "
+    "(fire.py:244-246 vs 342-345)",
+    "fire.fire_init / compute_cell_forces": "isotropic-stress closed forms for the "
+    "Frechet and unit filters; hydrostatic / constant-volume projections; Frechet "
+    "adjoint identity vs the 9-direction sum",
+    "gradients (models.pair_forward under torch.autograd, create_graph=True)": "central "
+    "finite differences of the force-matching loss and of the energy",
+    "NVE adjoint (autograd through md.nve_step)": "central finite differences of "
+    "E(x_T) wrt the pair parameters; checkpoint-interval invariance",
     "mat33.*": "scipy.linalg.expm / logm / expm_frechet",
 }

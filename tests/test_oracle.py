@@ -321,3 +321,235 @@ def test_fire_first_step_uses_raw_forces():
     f0 = st["forces"].clone()
     st = ofire.ase_fire_step(st, model)
     torch.testing.assert_close(st["velocities"], f0 * 0.1)
+
+
+# ---------------------------------------------------------------- FIRE closed forms
+def _const_force_model(force, stress=None):
+    """E = -sum F.x: constant forces, optional constant stress [S,3,3]."""
+    force = _t(force)
+
+    def fn(x, cell):
+        out = {"forces": force.clone(), "energy": -(force * x).sum().view(1)}
+        out["stress"] = torch.zeros(cell.shape[0], 3, 3, dtype=F64) if stress is None else stress.clone()
+        return out
+
+    return fn
+
+
+def _fire_state(force, v, n_pos, dt=0.1, alpha=0.1, flavor_vv=False):
+    pos = torch.zeros(len(force), 3, dtype=F64)
+    cell = (torch.eye(3, dtype=F64) * 10).unsqueeze(0)
+    idx = torch.zeros(len(force), dtype=torch.long)
+    model = _const_force_model(force)
+    st = ofire.fire_init(pos, cell, torch.ones(len(force), dtype=F64), idx, model, dt_start=dt,
+                         alpha_start=alpha)
+    st["velocities"] = _t(v)
+    st["n_pos"][:] = n_pos
+    return st, model
+
+
+def test_ase_fire_negative_power_branch():
+    """P < 0 (test_optimizers.py:600-662): dt *= f_dec, alpha = alpha_start,
+    n_pos = 0, velocities zeroed then v = dt_new F."""
+    f = [[0.3, -0.2, 0.1], [-0.1, 0.4, 0.2]]
+    st, model = _fire_state(f, (-0.1 * _t(f)).tolist(), n_pos=4, alpha=0.07)
+    st = ofire.ase_fire_step(st, model, max_step=10.0, alpha_start=0.1)
+    assert st["dt"].item() == pytest.approx(0.05, abs=1e-15)
+    assert st["alpha"].item() == pytest.approx(0.1, abs=1e-15)
+    assert st["n_pos"].item() == 0
+    torch.testing.assert_close(st["velocities"], 0.05 * _t(f), rtol=0, atol=1e-15)
+    torch.testing.assert_close(st["positions"], 0.05 * 0.05 * _t(f), rtol=0, atol=1e-15)
+
+
+def test_ase_fire_mixing_known_answer():
+    """One atom, v = (1,0,0), F = (2,2,0): P = 2 > 0, n_pos <= n_min so dt and
+    alpha stay; |v| F/|F| = (1,1,0)/sqrt(2); v' = 0.9 v + 0.1 (1,1,0)/sqrt(2);
+    then v += dt F and x += dt v."""
+    st, model = _fire_state([[2.0, 2.0, 0.0]], [[1.0, 0.0, 0.0]], n_pos=2)
+    st = ofire.ase_fire_step(st, model, max_step=10.0)
+    r = 1.0 / math.sqrt(2.0)
+    v = _t([[0.9 + 0.1 * r + 0.2, 0.1 * r + 0.2, 0.0]])
+    assert st["dt"].item() == 0.1 and st["alpha"].item() == 0.1 and st["n_pos"].item() == 3
+    torch.testing.assert_close(st["velocities"], v, rtol=0, atol=1e-15)
+    torch.testing.assert_close(st["positions"], 0.1 * v, rtol=0, atol=1e-15)
+    # two atoms: the norms are per system (|v| = 1, |F| = sqrt 2), not per atom
+    st, model = _fire_state([[1.0, 0.0, 0.0], [0.0, 1.0, 0.0]], [[1.0, 0.0, 0.0], [0.0, 0.0, 0.0]], n_pos=0)
+    st = ofire.ase_fire_step(st, model, max_step=10.0)
+    v = _t([[0.9 + 0.1 * r + 0.1, 0.0, 0.0], [0.0, 0.1 * r + 0.1, 0.0]])
+    torch.testing.assert_close(st["velocities"], v, rtol=0, atol=1e-15)
+
+
+def test_fire_mixing_leaves_parallel_velocity_unchanged():
+    """v = c F with c > 0 across a system: mixing with the system-wide norms
+    is the identity for any alpha (a per-atom norm would change it)."""
+    f = _t([[0.5, 0.0, 0.0], [0.0, 3.0, 0.0], [0.0, 0.0, 0.25]])
+    st, model = _fire_state(f.tolist(), (0.7 * f).tolist(), n_pos=9, dt=0.2, alpha=0.3)
+    st = ofire.ase_fire_step(st, model, max_step=100.0)
+    dt = 0.2 * 1.1
+    assert st["dt"].item() == pytest.approx(dt, rel=1e-15)
+    assert st["alpha"].item() == pytest.approx(0.3 * 0.99, rel=1e-15)
+    torch.testing.assert_close(st["velocities"], 0.7 * f + dt * f, rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("flavor", ["ase", "vv"])
+def test_fire_n_pos_ordering_differs_between_flavors(flavor):
+    """n_pos == n_min with P > 0: ASE tests n_pos > n_min before the increment
+    (fire.py:342-345, no dt growth); VV increments first (fire.py:244-246,
+    dt grows by f_inc)."""
+    f = [[1.0, 0.0, 0.0]]
+    st, model = _fire_state(f, [[1.0, 0.0, 0.0]], n_pos=5)
+    if flavor == "ase":
+        st = ofire.ase_fire_step(st, model)
+        assert st["dt"].item() == 0.1
+    else:
+        st = ofire.vv_fire_step(st, model)
+        assert st["dt"].item() == pytest.approx(0.11, rel=1e-15)
+    assert st["n_pos"].item() == 6
+
+
+def test_fire_dt_max_and_max_step_clamp():
+    """dt growth is capped at dt_max; a step longer than max_step is scaled to
+    exactly max_step along its own direction (per-system norm)."""
+    f = [[30.0, 40.0, 0.0], [0.0, 0.0, 0.0]]
+    st, model = _fire_state(f, [[3.0, 4.0, 0.0], [0.0, 0.0, 0.0]], n_pos=10, dt=0.95)
+    st = ofire.ase_fire_step(st, model, dt_max=1.0, max_step=0.2)
+    assert st["dt"].item() == 1.0
+    d = st["positions"]
+    assert d[0].norm().item() == pytest.approx(0.2, rel=1e-14)
+    torch.testing.assert_close(d[0] / d[0].norm(), _t([0.6, 0.8, 0.0]), rtol=0, atol=1e-15)
+    assert d[1].abs().max().item() == 0.0
+
+
+def test_vv_fire_negative_power_zeroes_velocities():
+    """VV flavour, P < 0 after the second half kick: velocities are zeroed,
+    dt *= f_dec, alpha = alpha_start (fire.py:240-262)."""
+    f = [[1.0, 0.0, 0.0]]
+    st, model = _fire_state(f, [[-5.0, 0.0, 0.0]], n_pos=3, alpha=0.05)
+    st = ofire.vv_fire_step(st, model)
+    assert st["dt"].item() == 0.05 and st["alpha"].item() == 0.1 and st["n_pos"].item() == 0
+    assert st["velocities"].abs().max().item() == 0.0
+    # x = dt (v + dt F / 2) with the old dt
+    torch.testing.assert_close(st["positions"], _t([[0.1 * (-5.0 + 0.05), 0.0, 0.0]]), rtol=0, atol=1e-15)
+
+
+def test_cell_filter_init_closed_forms():
+    """Isotropic stress p on a cubic cell of N atoms: virial = -V p I.
+    Frechet: F = I, log F = 0, so cell forces = virial / N; unit filter:
+    cell positions = N I, cell forces = virial / N.  Hydrostatic projection
+    keeps the mean diagonal, constant volume removes it."""
+    n, a, p = 3, 4.0, 0.02
+    stress = (p * torch.eye(3, dtype=F64)).unsqueeze(0)
+    pos = torch.zeros(n, 3, dtype=F64)
+    cell = (torch.eye(3, dtype=F64) * a).unsqueeze(0)
+    idx = torch.zeros(n, dtype=torch.long)
+    model = _const_force_model(torch.zeros(n, 3).tolist(), stress)
+    want = -(a ** 3) * p * torch.eye(3, dtype=F64) / n
+    for kw in ({"frechet": True}, {"unit": True}):
+        st = ofire.fire_init(pos, cell, torch.ones(n, dtype=F64), idx, model, **kw)
+        torch.testing.assert_close(st["cell_forces"][0], want, rtol=1e-15, atol=0)
+    assert torch.equal(st["cell_positions"][0], n * torch.eye(3, dtype=F64))
+    v = _t([[[1.0, 2.0, 3.0], [4.0, 5.0, 6.0], [7.0, 8.0, 9.0]]])
+    h = ofire._apply_constraints(v, {"hydrostatic_strain": True})
+    torch.testing.assert_close(h[0], 5.0 * torch.eye(3, dtype=F64))
+    c = ofire._apply_constraints(v, {"constant_volume": True})
+    assert torch.diagonal(c[0]).sum().item() == pytest.approx(0.0, abs=1e-14)
+    torch.testing.assert_close(c[0] - v[0], -5.0 * torch.eye(3, dtype=F64))
+
+
+def test_frechet_first_step_scales_cell_by_scalar_exp():
+    """Zero atomic forces, isotropic stress: the first ASE step moves the cell
+    log-strain by dt^2 * c with c the (isotropic) cell force, so the cell and
+    every position scale by exp(dt^2 c / N)."""
+    n, a, p, dt = 4, 5.0, -0.01, 0.1
+    stress = (p * torch.eye(3, dtype=F64)).unsqueeze(0)
+    pos = _t([[0.5, 1.0, 1.5], [2.0, 0.2, 3.0], [4.0, 4.5, 0.1], [1.0, 3.0, 2.0]])
+    cell = (torch.eye(3, dtype=F64) * a).unsqueeze(0)
+    idx = torch.zeros(n, dtype=torch.long)
+    model = _const_force_model(torch.zeros(n, 3).tolist(), stress)
+    st = ofire.fire_init(pos, cell, torch.ones(n, dtype=F64), idx, model, frechet=True, dt_start=dt)
+    c = -(a ** 3) * p / n  # isotropic cell force
+    st = ofire.ase_fire_step(st, model, max_step=10.0)
+    s = math.exp(dt * dt * c / n)
+    torch.testing.assert_close(st["cell"][0], a * s * torch.eye(3, dtype=F64), rtol=1e-14, atol=0)
+    torch.testing.assert_close(st["positions"], pos * s, rtol=1e-14, atol=1e-15)
+    torch.testing.assert_close(st["cell_positions"][0], dt * dt * c * torch.eye(3, dtype=F64),
+                               rtol=1e-12, atol=1e-16)
+
+
+# ---------------------------------------------------------------- gradients
+KA_SIG = torch.tensor([[1.0, 0.8], [0.8, 0.88]], dtype=F64)
+KA_EPS = torch.tensor([[1.0, 1.5], [1.5, 0.5]], dtype=F64)
+
+
+def _binary_lj(seed=0, reps=3, rho=0.8):
+    a = (4.0 / rho) ** (1.0 / 3.0)
+    g = torch.Generator().manual_seed(seed)
+    pos = fcc(a, [reps] * 3)
+    pos = pos + 0.05 * torch.randn(pos.shape, generator=g, dtype=F64)
+    n = pos.shape[0]
+    cell = (torch.eye(3, dtype=F64) * a * reps).unsqueeze(0)
+    return pos, cell, torch.tensor([1, 2] * (n // 2)), torch.zeros(n, dtype=torch.long)
+
+
+def _ka_forward(pos, cell, z, idx, s, e, create_graph):
+    fn = om.MultiPairFn([1, 2], s, e, kind="lennard_jones")
+    return om.pair_forward(pos, cell, torch.tensor([True] * 3), z, idx, fn, 2.5,
+                           compute_stress=False, create_graph=create_graph, nl=onl.linked_cell_nl)
+
+
+def _fd_check(loss_fn, gs, ge, h=1e-6, rel=1e-6):
+    """Central differences of a scalar loss(sigma, eps) against autograd; an
+    off-diagonal parameter is shared, so it moves both entries."""
+    for a, b in [(0, 0), (0, 1), (1, 1)]:
+        dp = torch.zeros(2, 2, dtype=F64)
+        dp[a, b] = dp[b, a] = h
+        for g, fd in ((gs, (loss_fn(KA_SIG + dp, KA_EPS) - loss_fn(KA_SIG - dp, KA_EPS)) / (2 * h)),
+                      (ge, (loss_fn(KA_SIG, KA_EPS + dp) - loss_fn(KA_SIG, KA_EPS - dp)) / (2 * h))):
+            got = (g[a, b] + g[b, a]).item() if a != b else g[a, b].item()
+            assert abs(got - fd) <= rel * max(abs(fd), 1e-3), (a, b, got, fd)
+
+
+def test_force_matching_autograd_matches_finite_differences():
+    """The gradient oracle (autograd with create_graph through the forces)
+    against FP64 central differences of the force-matching loss."""
+    pos, cell, z, idx = _binary_lj()
+    target = _ka_forward(pos, cell, z, idx, KA_SIG * 1.03, KA_EPS * 0.95, False)["forces"]
+
+    def loss(s, e, cg=False):
+        x = pos.clone().requires_grad_(cg)  # the force graph runs through the positions
+        return ((_ka_forward(x, cell, z, idx, s, e, cg)["forces"] - target) ** 2).mean()
+
+    s, e = KA_SIG.clone().requires_grad_(True), KA_EPS.clone().requires_grad_(True)
+    gs, ge = torch.autograd.grad(loss(s, e, True), [s, e])
+    _fd_check(lambda a, b: loss(a, b).item(), gs, ge)
+
+
+def test_nve_adjoint_oracle_matches_finite_differences():
+    """Autograd through 8 velocity-Verlet steps of the oracle: dE(x_T)/dtheta
+    and dE(x_T)/dx0 against central differences."""
+    pos, cell, z, idx = _binary_lj(seed=1, reps=2, rho=0.5)
+    m = torch.ones(pos.shape[0], dtype=F64)
+    g = torch.Generator().manual_seed(3)
+    p0 = 0.5 * torch.randn(pos.shape, generator=g, dtype=F64)
+
+    def final_energy(x0, s, e, cg):
+        def force_fn(x):
+            out = _ka_forward(x, cell, z, idx, s, e, cg)
+            return out["energy"], out["forces"]
+
+        x, p = x0, p0
+        _, f = force_fn(x)
+        for _ in range(8):
+            x, p, f, _ = omd.nve_step(x, p, f, m, force_fn, 0.004)
+        return force_fn(x)[0].sum()
+
+    x0 = pos.clone().requires_grad_(True)
+    s, e = KA_SIG.clone().requires_grad_(True), KA_EPS.clone().requires_grad_(True)
+    gx, gs, ge = torch.autograd.grad(final_energy(x0, s, e, True), [x0, s, e])
+    _fd_check(lambda a, b: final_energy(pos, a, b, False).item(), gs, ge)
+    h = 1e-6
+    for (i, c) in [(0, 0), (5, 2), (17, 1)]:
+        d = torch.zeros_like(pos)
+        d[i, c] = h
+        fd = (final_energy(pos + d, KA_SIG, KA_EPS, False) - final_energy(pos - d, KA_SIG, KA_EPS, False)).item() / (2 * h)
+        assert abs(gx[i, c].item() - fd) <= 1e-6 * max(abs(fd), 1e-3), (i, c, gx[i, c].item(), fd)
