@@ -89,7 +89,8 @@ const char* tsx_last_error(void);
  *     nbr[(n / tpa) * (ld * tpa) + t * tpa + (n % tpa)]
  * and packs the neighbor's system-local index in the low ``jbits`` bits and
  * the unwrapped-frame image shift S' = S + w_i - w_j (valid for the input
- * positions) as three ``sbits``-bit offset-binary fields above it.
+ * positions) as three ``sbits``-bit two's-complement fields above it
+ * (sbits = (32 - jbits) / 3; a zero shift is a zero high part).
  * ``count[t]`` is the number of entries; overflowing rows set
  * TSX_STATUS_NL_OVERFLOW and are truncated at kmax.
  *
