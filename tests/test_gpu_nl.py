@@ -117,25 +117,77 @@ def test_layouts_and_bin_sizes_give_same_set(dev, tpa, q):
     _assert_same(onl.sort_pairs(*(t.cpu() for t in nl.pairs(si, F64))), ref)
 
 
+def _raw_rows(nl, n):
+    """Every row of the ELL list as (count, entries) in thread order."""
+    cnt = nl.count[:n].cpu()
+    nbr = nl.nbr.cpu()
+    rows = []
+    for t in range(n):
+        c = int(cnt[t])
+        idx = [(k // nl.tpa) * (nl.ld * nl.tpa) + t * nl.tpa + k % nl.tpa for k in range(c)]
+        rows.append(nbr[idx].tolist())
+    return nl.order[:n].cpu().tolist(), rows
+
+
 @pytest.mark.parametrize("q", [1, 2])
 def test_fill_kernels_write_identical_rows(dev, q):
-    """The per-bin (LDS-staged) and per-atom fill kernels write every row with
-    the same neighbours in the same order (stencil image, then atom index), on
-    small triclinic cells (many periodic images) and larger ones."""
+    """The per-bin and per-atom fill kernels write every row with the same
+    neighbour words in the same order (stencil image, then atom index), on
+    small triclinic cells (many periodic images) and larger ones, full and
+    half lists."""
     st = tsa.workloads.cu_batch(6, device="cpu", seed=8)
     small = tsa.workloads.cu_batch(3, device="cpu", reps=[[2, 2, 2], [1, 1, 1], [1, 2, 3]], seed=2)
     tri = tsa.workloads.soft_sphere_batch(4, n_min=40, n_max=120, device="cpu", seed=3)
     for s_ in (st, small, tri):
         si = SystemIndex(s_.system_idx.cuda(), s_.n_systems)
-        outs = []
-        for mode in (1, 2):
-            nl = NeighborList(6.0, tpa=4, bins_per_rlist=q, fill_mode=mode)
-            nl.build(s_.positions.cuda(), s_.cell.cuda().contiguous(), s_.pbc.cuda(), si)
-            outs.append([t.cpu() for t in nl.pairs(si, F64)])
-        for a, b in zip(*outs):
-            assert torch.equal(a, b)
+        for half in (False, True):
+            outs = []
+            for mode in (1, 2):
+                print("fill mode", mode, "q", q, "half", half, "atoms", s_.n_atoms, flush=True)
+                nl = NeighborList(6.0, half=half, tpa=4, bins_per_rlist=q, fill_mode=mode)
+                nl.build(s_.positions.cuda(), s_.cell.cuda().contiguous(), s_.pbc.cuda(), si)
+                outs.append(_raw_rows(nl, s_.n_atoms))
+            assert outs[0] == outs[1]
+        nl = NeighborList(6.0, tpa=4, bins_per_rlist=q, fill_mode=1)
+        nl.build(s_.positions.cuda(), s_.cell.cuda().contiguous(), s_.pbc.cuda(), si)
         ref = onl.brute_force_nl(s_.positions, s_.cell.mT, s_.pbc, 6.0, s_.system_idx)
-        _assert_same(onl.sort_pairs(*outs[0]), ref)
+        _assert_same(onl.sort_pairs(*(t.cpu() for t in nl.pairs(si, F64))), ref)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_fill_exact_on_edge_cases(dev, mode):
+    """Both fill kernels against brute force: unwrapped input ~140 cells from
+    the cell (wrap counts too large for the unchecked word path, so rows take
+    the checked path while every S' still fits), self interaction, per-system
+    pbc, and a dense list (rc = 11 A in Cu, ~470 neighbours)."""
+    pos, cell, pbc, idx = _cacrp2o7()
+    n = pos.shape[0]
+    ar = torch.arange(n)
+    far = torch.stack([(ar % 3) + 139, (ar % 2) - 136, (ar % 3) + 149], 1).to(F64)
+    shifted = pos + far @ cell[0]
+    for p_, si_ in ((shifted, False), (pos, True)):
+        print("fill mode", mode, "cacrp2o7 self_interaction", si_, flush=True)
+        ref = onl.brute_force_nl(p_, cell, pbc, 5.0, idx, si_)
+        sidx = SystemIndex(idx.cuda(), 1)
+        nl = NeighborList(5.0, self_interaction=si_, tpa=2, bins_per_rlist=2, fill_mode=mode)
+        nl.build(p_.cuda(), cell.transpose(1, 2).contiguous().cuda(), pbc.cuda(), sidx)
+        _assert_same(onl.sort_pairs(*(t.cpu() for t in nl.pairs(sidx, F64))), ref)
+    st = tsa.workloads.soft_sphere_batch(3, n_min=20, n_max=40, device="cpu", seed=7)
+    pb = torch.tensor([[True, True, True], [True, False, True], [False, False, False]])
+    ref = onl.brute_force_nl(st.positions, st.cell.mT, pb, 1.8, st.system_idx)
+    sidx = SystemIndex(st.system_idx.cuda(), st.n_systems)
+    print("fill mode", mode, "per-system pbc", flush=True)
+    nl = NeighborList(1.8, tpa=2, bins_per_rlist=2, fill_mode=mode)
+    nl.build(st.positions.cuda(), st.cell.cuda().contiguous(), pb.cuda(), sidx)
+    _assert_same(onl.sort_pairs(*(t.cpu() for t in nl.pairs(sidx, F64))), ref)
+    big = tsa.workloads.cu_batch(1, device="cpu", reps=[[7, 7, 7]], seed=3)
+    ref = onl.sort_pairs(*onl.linked_cell_nl(big.positions, big.cell.mT, big.pbc, 11.0,
+                                             big.system_idx))
+    sidx = SystemIndex(big.system_idx.cuda(), 1)
+    print("fill mode", mode, "dense Cu rc 11", flush=True)
+    nl = NeighborList(11.0, tpa=4, bins_per_rlist=2, fill_mode=mode)
+    nl.build(big.positions.cuda(), big.cell.cuda().contiguous(), big.pbc.cuda(), sidx)
+    _assert_same(onl.sort_pairs(*(t.cpu() for t in nl.pairs(sidx, F64))), ref)
 
 
 def test_wrap_positions_matches_oracle(dev):

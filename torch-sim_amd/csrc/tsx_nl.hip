@@ -56,6 +56,8 @@ struct NLWs {
   int32_t* scan_tmp;   // scan block sums
   int32_t* unsorted;   // N: atoms scattered to their bins, before the in-bin sort
   int32_t* act;        // [S + 1] exclusive prefix of the bins of gated-on systems
+  uint32_t* cw;        // N, bin order: partial list word (local index, negated wrap)
+  int32_t* wmax;       // S: largest wrap-shift magnitude of the system's atoms
 };
 
 static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -79,6 +81,8 @@ static size_t ws_layout(int64_t n, int32_t s, int32_t bin_cap, NLWs* w, char* ba
   void* p_st = take(sizeof(int32_t) * 4096);
   void* p_un = take(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
   void* p_act = take(sizeof(int32_t) * ((size_t)(s > 0 ? s : 1) + 1));
+  void* p_cw = take(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+  void* p_wm = take(sizeof(int32_t) * (size_t)(s > 0 ? s : 1));
   if (w) {
     w->geom = (double*)p_geom;
     w->grid = (int32_t*)p_grid;
@@ -91,6 +95,8 @@ static size_t ws_layout(int64_t n, int32_t s, int32_t bin_cap, NLWs* w, char* ba
     w->scan_tmp = (int32_t*)p_st;
     w->unsorted = (int32_t*)p_un;
     w->act = (int32_t*)p_act;
+    w->cw = (uint32_t*)p_cw;
+    w->wmax = (int32_t*)p_wm;
   }
   return off;
 }
@@ -108,6 +114,19 @@ __device__ __forceinline__ void unpack_w(double d, int* w) {
   w[0] = (int)(v & m) - (1 << 20);
   w[1] = (int)((v >> 21) & m) - (1 << 20);
   w[2] = (int)((v >> 42) & m) - (1 << 20);
+}
+
+// Three two's-complement sbits-wide fields (x, y, z), low field first.
+__device__ __forceinline__ uint32_t pack_fields(int x, int y, int z, int sbits) {
+  const uint32_t m = (1u << sbits) - 1u;
+  return ((uint32_t)x & m) | (((uint32_t)y & m) << sbits) | (((uint32_t)z & m) << (2 * sbits));
+}
+
+// Field-wise sum of two packed words, modulo 2^sbits per field: the low bits
+// of every field are added with the top bits (H) cleared, so no carry leaves
+// a field, and the top bits are added by xor.
+__device__ __forceinline__ uint32_t field_add(uint32_t a, uint32_t b, uint32_t H) {
+  return ((a & ~H) + (b & ~H)) ^ ((a ^ b) & H);
 }
 
 __device__ __forceinline__ bool pbc_of(const uint8_t* pbc, int per_sys, int s, int k) {
@@ -428,11 +447,12 @@ __global__ __launch_bounds__(256) void k_scan_sys(
     const int32_t* __restrict__ sys_start, const int32_t* __restrict__ bin_count,
     int32_t* __restrict__ bin_start, const int32_t* __restrict__ gate,
     int32_t* __restrict__ counter, const double* __restrict__ cell_col,
-    double* __restrict__ cell_build) {
+    double* __restrict__ cell_build, int32_t* __restrict__ wmax) {
   __shared__ int32_t wsum[4];
   __shared__ int32_t carry;
   const int s = blockIdx.x;
   if (meta[0] > bin_cap || gated_off(gate, s)) return;
+  if (threadIdx.x == 0) wmax[s] = 0;  // raised by the sort of this build
   const int32_t* gr = grid + (size_t)GRID * s;
   const int off = gr[6], nb = gr[0] * gr[1] * gr[2];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -490,21 +510,28 @@ __global__ __launch_bounds__(256) void k_scatter(
 // One thread per atom of a rebuilt system: its rank among the atoms of its
 // bin (by atom index, so the order is unique and does not depend on the
 // scatter's atomic slots), its place order[bin_start + rank], its wrapped
-// coordinates at that place, and the bin counter reset for the next build.
-// Single-bin systems were scattered in atom order already.
+// coordinates at that place, its partial list word cw (system-local index in
+// the low jbits, negated wrap shift in the three shift fields: a neighbour's
+// word is this plus the image shift and the centre's wrap shift, field by
+// field), the largest wrap magnitude of the system, and the bin counter
+// reset for the next build.  Single-bin systems were scattered in atom order
+// already.
 __global__ __launch_bounds__(256) void k_sort_atom(
     const double* __restrict__ pos, int ps, const int32_t* __restrict__ sys_start, int n_sys,
     int B, const double* __restrict__ geom, const int32_t* __restrict__ grid,
     const int32_t* __restrict__ meta, int bin_cap, const int32_t* __restrict__ atom_bin,
     const int32_t* __restrict__ bin_start, const int32_t* __restrict__ unsorted,
     int32_t* __restrict__ bin_count, int32_t* __restrict__ order, double* __restrict__ wr,
-    const int32_t* __restrict__ gate) {
+    const int32_t* __restrict__ gate, int jbits, uint32_t* __restrict__ cw,
+    int32_t* __restrict__ wmax) {
   const int s = blockIdx.x / B, part = blockIdx.x % B;
   if (s >= n_sys || meta[0] > bin_cap || gated_off(gate, s)) return;
   const int32_t* gr = grid + (size_t)GRID * s;
   const double* g = geom + (size_t)GEOM * s;
   const bool single = gr[0] * gr[1] * gr[2] == 1;
   const int e = sys_start[s + 1];
+  const int sbits = (32 - jbits) / 3;
+  int wm = 0;
   for (int i = sys_start[s] + part * blockDim.x + threadIdx.x; i < e; i += B * blockDim.x) {
     const int b = atom_bin[i];
     const int lo = bin_start[b], hi = bin_start[b + 1];
@@ -524,8 +551,13 @@ __global__ __launch_bounds__(256) void k_sort_atom(
     v.z = o.wr[2];
     v.w = pack_w(o.w[0], o.w[1], o.w[2]);
     reinterpret_cast<double4*>(wr)[dst] = v;
+    cw[dst] = (uint32_t)(i - sys_start[s]) |
+              (pack_fields(-o.w[0], -o.w[1], -o.w[2], sbits) << jbits);
+    wm = max(wm, max(abs(o.w[0]), max(abs(o.w[1]), abs(o.w[2]))));
     bin_count[b] = 0;
   }
+  for (int off = 32; off > 0; off >>= 1) wm = max(wm, __shfl_xor(wm, off, 64));
+  if ((threadIdx.x & 63) == 0 && wm > 0) atomicMax(&wmax[s], wm);
 }
 
 __device__ __forceinline__ int floordiv(int a, int b) {
@@ -551,6 +583,79 @@ __device__ __forceinline__ int64_t ell_index(int64_t t, int n, int tpa, int64_t 
 }
 
 
+// Component m of the image shift S . L (row cell g) as one FMA chain, the
+// same in every fill kernel so that they all write identical rows.
+__device__ __forceinline__ double shift_comp(int sx, int sy, int sz, const double* g, int m) {
+  return fma((double)sz, g[6 + m], fma((double)sy, g[3 + m], (double)sx * g[m]));
+}
+
+// Largest |S_k| over the stencil images of any bin of a system.
+__device__ __forceinline__ int stencil_smax(const int32_t* gr) {
+  int r = 0;
+  for (int k = 0; k < 3; ++k) {
+    if (gr[3 + k] == 0) continue;
+    const int lo = -floordiv(-gr[3 + k], gr[k]);
+    const int hi = floordiv(gr[k] - 1 + gr[3 + k], gr[k]);
+    r = max(r, max(lo, hi));
+  }
+  return r;
+}
+
+// Per-system constants of the list word.  A system is "fast" when its image
+// shifts and wrap shifts cannot push any field of S' = S + w_i - w_j out of
+// range; its words are then field_add(cw_j, S + w_i) with no checks.
+struct WordCtx {
+  int jbits, sbits, bias;
+  uint32_t jmask, H;
+};
+
+__device__ __forceinline__ WordCtx word_ctx(int jbits) {
+  WordCtx c;
+  c.jbits = jbits;
+  c.sbits = (32 - jbits) / 3;
+  c.bias = 1 << (c.sbits - 1);
+  c.jmask = (1u << jbits) - 1u;
+  c.H = pack_fields(c.bias, c.bias, c.bias, c.sbits) << jbits;
+  return c;
+}
+
+// The checked word of one candidate (any wrap magnitude): returns false and
+// sets ovf when a field of S' does not fit.
+__device__ __forceinline__ bool checked_word(const WordCtx& c, int jl, int sx, int sy, int sz,
+                                             const int* w_i, double wj_w, bool half, int il,
+                                             bool& match, uint32_t& word, bool& ovf) {
+  int w_j[3];
+  unpack_w(wj_w, w_j);
+  const int ux = sx + w_i[0] - w_j[0];
+  const int uy = sy + w_i[1] - w_j[1];
+  const int uz = sz + w_i[2] - w_j[2];
+  if (match && half) {
+    if (il != jl) {
+      match = il < jl;
+    } else {
+      const int first = ux != 0 ? ux : (uy != 0 ? uy : uz);
+      match = first > 0;
+    }
+  }
+  if (match && (ux < -c.bias || ux >= c.bias || uy < -c.bias || uy >= c.bias || uz < -c.bias ||
+                uz >= c.bias)) {
+    ovf = true;
+    match = false;
+  }
+  word = nl_word(jl, ux, uy, uz, c.jbits, c.sbits);
+  return match;
+}
+
+// The half-list rule on a fast word (self images: first nonzero shift > 0).
+__device__ __forceinline__ bool half_keep(const WordCtx& c, uint32_t word, int il, int jl) {
+  if (il != jl) return il < jl;
+  const int ux = nl_shift(word, c.jbits, c.sbits, 0);
+  const int uy = nl_shift(word, c.jbits, c.sbits, 1);
+  const int uz = nl_shift(word, c.jbits, c.sbits, 2);
+  const int first = ux != 0 ? ux : (uy != 0 ? uy : uz);
+  return first > 0;
+}
+
 // ------------------------------------------------------------------- fill
 // Persistent grid of four independent waves per block walking the active-bin
 // list (a wave takes one bin at a time).
@@ -571,31 +676,36 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+template <bool HALF>
 __global__ __launch_bounds__(64 * FILL_W) void k_fill(
     const int32_t* __restrict__ sys_start, int n_sys, const double* __restrict__ geom,
     const int32_t* __restrict__ grid, const int32_t* __restrict__ meta, int bin_cap,
     const int32_t* __restrict__ act, const int32_t* __restrict__ bin_start,
-    const int32_t* __restrict__ order, const double* __restrict__ wr, double rl2, int self_int,
-    int half, int jbits, int tpa, int kmax, int64_t ld, uint32_t* __restrict__ nbr,
+    const int32_t* __restrict__ order, const double* __restrict__ wr,
+    const uint32_t* __restrict__ cw, const int32_t* __restrict__ wmax, double rl2, int self_int,
+    int jbits, int lt, int kmax, int64_t ld, uint32_t* __restrict__ nbr,
     int32_t* __restrict__ count, int32_t* __restrict__ status) {
   __shared__ double4 cpos_all[FILL_W][FILL_CH];
-  __shared__ int32_t cj_all[FILL_W][FILL_CH];
+  __shared__ uint32_t ccw_all[FILL_W][FILL_CH];
   __shared__ int32_t cimg_all[FILL_W][FILL_CH];
   __shared__ double ics_all[FILL_W][FILL_IMG][3];
   __shared__ int32_t is_all[FILL_W][FILL_IMG][3];
+  __shared__ uint32_t sw_all[FILL_W][FILL_IMG];
   __shared__ int32_t ibeg_all[FILL_W][FILL_IMG];
   __shared__ int32_t ipre_all[FILL_W][FILL_IMG + 1];
   if (meta[0] > bin_cap) return;
   const int wv = threadIdx.x >> 6, tid = threadIdx.x & 63;
   double4* cpos = cpos_all[wv];
-  int32_t* cj = cj_all[wv];
+  uint32_t* ccw = ccw_all[wv];
   int32_t* cimg = cimg_all[wv];
   double(*ics)[3] = ics_all[wv];
   int32_t(*isv)[3] = is_all[wv];
+  uint32_t* sw = sw_all[wv];
   int32_t* ibeg = ibeg_all[wv];
   int32_t* ipre = ipre_all[wv];
-  const int sbits = (32 - jbits) / 3;
-  const int bias = 1 << (sbits - 1);
+  const WordCtx wc = word_ctx(jbits);
+  const int64_t stride = ld << lt;
+  const int tmask = (1 << lt) - 1;
   const int total = act[n_sys];
   for (int wi_ = blockIdx.x * FILL_W + wv; wi_ < total; wi_ += gridDim.x * FILL_W) {
     const int s = active_system(act, n_sys, wi_);
@@ -609,6 +719,7 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
     const int c0 = bin_start[b], c1 = bin_start[b + 1];
     const int nc = c1 - c0;
     if (nc == 0) continue;
+    const bool fast = stencil_smax(gr) + 2 * wmax[s] < wc.bias;
     const int local = b - off;
     const int bz0 = local % gr[2], by0 = (local / gr[2]) % gr[1], bx0 = local / (gr[1] * gr[2]);
     int per = 1;  // centre atoms per round (power of two)
@@ -627,6 +738,9 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
         i = order[t];
         unpack_w(wi.w, w_i);
       }
+      const int il = i - base;
+      const uint32_t wiw = pack_fields(w_i[0], w_i[1], w_i[2], wc.sbits) << jbits;
+      const int64_t rowbase = (int64_t)t << lt;
       int cnt = 0;
       bool shift_ovf = false;
       for (int ib0 = 0; ib0 < n_img; ib0 += FILL_IMG) {
@@ -644,13 +758,14 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
           const int sx = floordiv(cx, gr[0]), sy = floordiv(cy, gr[1]), sz = floordiv(cz, gr[2]);
           const int nbin = off + ((cx - sx * gr[0]) * gr[1] + (cy - sy * gr[1])) * gr[2] +
                            (cz - sz * gr[2]);
-          // S . L in the reference einsum order (transforms.py:563-565)
-          ics[q][0] = (double)sx * g[0] + (double)sy * g[3] + (double)sz * g[6];
-          ics[q][1] = (double)sx * g[1] + (double)sy * g[4] + (double)sz * g[7];
-          ics[q][2] = (double)sx * g[2] + (double)sy * g[5] + (double)sz * g[8];
+          // S . L (transforms.py:563-565)
+          ics[q][0] = shift_comp(sx, sy, sz, g, 0);
+          ics[q][1] = shift_comp(sx, sy, sz, g, 1);
+          ics[q][2] = shift_comp(sx, sy, sz, g, 2);
           isv[q][0] = sx;
           isv[q][1] = sy;
           isv[q][2] = sz;
+          sw[q] = pack_fields(sx, sy, sz, wc.sbits) << jbits;
           const int kb = bin_start[nbin], ke = bin_start[nbin + 1];
           ibeg[q] = kb;
           ipre[q + 1] = ke - kb;
@@ -683,7 +798,7 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
             }
             const int k = ibeg[lo] + (gc - ipre[lo]);
             cpos[c] = reinterpret_cast<const double4*>(wr)[k];
-            cj[c] = order[k];
+            ccw[c] = cw[k];
             cimg[c] = lo;
           }
           wave_sync();
@@ -697,34 +812,24 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
               const double d2 =
                   dist2_ref(wi.x, wi.y, wi.z, wj.x, wj.y, wj.z, ics[q][0], ics[q][1], ics[q][2]);
               if (d2 < rl2) {
-                const int j = cj[c];
-                const int sx = isv[q][0], sy = isv[q][1], sz = isv[q][2];
-                match = !(j == i && (sx | sy | sz) == 0 && !self_int);
-                int w_j[3];
-                unpack_w(wj.w, w_j);
-                const int ux = sx + w_i[0] - w_j[0];
-                const int uy = sy + w_i[1] - w_j[1];
-                const int uz = sz + w_i[2] - w_j[2];
-                if (match && half) {
-                  if (i != j) {
-                    match = i < j;
-                  } else {
-                    const int first = ux != 0 ? ux : (uy != 0 ? uy : uz);
-                    match = first > 0;
-                  }
+                const uint32_t cwj = ccw[c];
+                const int jl = (int)(cwj & wc.jmask);
+                if (fast) {
+                  const uint32_t swq = sw[q];
+                  word = (uint32_t)jl | field_add(cwj & ~wc.jmask, field_add(swq, wiw, wc.H), wc.H);
+                  match = self_int || !(swq == 0 && jl == il);
+                  if (HALF && match) match = half_keep(wc, word, il, jl);
+                } else {
+                  const int sx = isv[q][0], sy = isv[q][1], sz = isv[q][2];
+                  match = self_int || !(jl == il && (sx | sy | sz) == 0);
+                  checked_word(wc, jl, sx, sy, sz, w_i, wj.w, HALF, il, match, word, shift_ovf);
                 }
-                if (match && (ux < -bias || ux >= bias || uy < -bias || uy >= bias ||
-                              uz < -bias || uz >= bias)) {
-                  shift_ovf = true;
-                  match = false;
-                }
-                word = nl_word(j - base, ux, uy, uz, jbits, sbits);
               }
             }
             const uint64_t bal = __ballot(match);
             if (match) {
               const int pos_n = cnt + __popcll(bal & below);
-              if (pos_n < kmax) nbr[ell_index(t, pos_n, tpa, ld)] = word;
+              if (pos_n < kmax) nbr[rowbase + (int64_t)(pos_n >> lt) * stride + (pos_n & tmask)] = word;
             }
             cnt += __popcll(bal & gmask);
           }
@@ -749,18 +854,20 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
 // group walks the stencil images of the atom's bin in order; within a
 // candidate range lane `sub` tests candidates kb + sub, kb + sub + L, ...; a
 // ballot over the group compacts the matches, so the row is written in
-// candidate order (the order a single thread would meet them).  Candidates are read from L2:
-// rows of one bin share their candidates, so the reads coalesce.  No LDS and
-// no staging, so sparse bins cost only their candidates, and partial
-// (gated) rebuilds still fill the chip with L lanes per rebuilt atom.
-template <int L>
+// candidate order (the order a single thread would meet them).  Candidates
+// are read from L2: rows of one bin share their candidates, so the reads
+// coalesce.  No LDS and no staging, so sparse bins cost only their
+// candidates, and partial (gated) rebuilds still fill the chip with L lanes
+// per rebuilt atom.
+template <int L, bool HALF>
 __global__ __launch_bounds__(256) void k_fill_atom(
     const int32_t* __restrict__ sys_start, const int32_t* __restrict__ atom_sys,
     int64_t n_atoms, const double* __restrict__ geom, const int32_t* __restrict__ grid,
     const int32_t* __restrict__ meta, int bin_cap, const int32_t* __restrict__ atom_bin,
     const int32_t* __restrict__ bin_start, const int32_t* __restrict__ order,
-    const double* __restrict__ wr, double rl2, int self_int, int half, int jbits, int tpa,
-    int kmax, int64_t ld, uint32_t* __restrict__ nbr, int32_t* __restrict__ count,
+    const double* __restrict__ wr, const uint32_t* __restrict__ cw,
+    const int32_t* __restrict__ wmax, double rl2, int self_int, int jbits, int lt, int kmax,
+    int64_t ld, uint32_t* __restrict__ nbr, int32_t* __restrict__ count,
     int32_t* __restrict__ status, const int32_t* __restrict__ gate) {
   const int64_t gid = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   const int64_t t = gid / L;
@@ -776,16 +883,21 @@ __global__ __launch_bounds__(256) void k_fill_atom(
   const double* g = geom + (size_t)GEOM * s;
   const int32_t* gr = grid + (size_t)GRID * s;
   const int base = sys_start[s];
+  const int il = i - base;
   const int off = gr[6];
   const int nbx = gr[0], nby = gr[1], nbz = gr[2];
   const int mx = gr[3], my = gr[4], mz = gr[5];
   const int local = atom_bin[i] - off;
   const int bz0 = local % nbz, by0 = (local / nbz) % nby, bx0 = local / (nby * nbz);
-  const int sbits = (32 - jbits) / 3;
-  const int bias = 1 << (sbits - 1);
+  const WordCtx wc = word_ctx(jbits);
+  const bool fast = stencil_smax(gr) + 2 * wmax[s] < wc.bias;
   const double4 wi = reinterpret_cast<const double4*>(wr)[t];
   int w_i[3];
   unpack_w(wi.w, w_i);
+  const uint32_t wiw = pack_fields(w_i[0], w_i[1], w_i[2], wc.sbits) << jbits;
+  const int64_t rowbase = t << lt;
+  const int64_t stride = ld << lt;
+  const int tmask = (1 << lt) - 1;
   int cnt = 0;
   bool shift_ovf = false;
   // Stencil columns (dx, dy); along z the bins of a column are contiguous in
@@ -808,60 +920,42 @@ __global__ __launch_bounds__(256) void k_fill_atom(
         cz += run;
         const int kb = bin_start[colbase + iz], ke = bin_start[colbase + iz + run];
         if (kb == ke) continue;
-        // S . L in the reference einsum order (transforms.py:563-565)
-        const double c0 = (double)sx * g[0] + (double)sy * g[3] + (double)sz * g[6];
-        const double c1 = (double)sx * g[1] + (double)sy * g[4] + (double)sz * g[7];
-        const double c2 = (double)sx * g[2] + (double)sy * g[5] + (double)sz * g[8];
-        // two candidates per lane in flight: both loads issue before either test
+        const double c0 = shift_comp(sx, sy, sz, g, 0);
+        const double c1 = shift_comp(sx, sy, sz, g, 1);
+        const double c2 = shift_comp(sx, sy, sz, g, 2);
+        const uint32_t iw = field_add(pack_fields(sx, sy, sz, wc.sbits) << jbits, wiw, wc.H);
+        const bool zimg = (sx | sy | sz) == 0;
+        // two candidates per lane in flight; out-of-range lanes re-read the
+        // run's first candidate, so every load is unconditional
         for (int k0 = kb; k0 < ke; k0 += 2 * L) {
           const int ka = k0 + sub, kc = k0 + L + sub;
-          double4 wa = make_double4(0.0, 0.0, 0.0, 0.0), wc = wa;
-          int ja = 0, jc = 0;
-          if (ka < ke) {
-            wa = reinterpret_cast<const double4*>(wr)[ka];
-            ja = order[ka];
-          }
-          if (kc < ke) {
-            wc = reinterpret_cast<const double4*>(wr)[kc];
-            jc = order[kc];
-          }
+          const bool va = ka < ke, vc = kc < ke;
+          const int kar = va ? ka : kb, kcr = vc ? kc : kb;
+          const double4 wa = reinterpret_cast<const double4*>(wr)[kar];
+          const double4 wb = reinterpret_cast<const double4*>(wr)[kcr];
+          const uint32_t ca = cw[kar], cb = cw[kcr];
 #pragma unroll
           for (int u = 0; u < 2; ++u) {
-            const int k = u == 0 ? ka : kc;
-            const double4 wj = u == 0 ? wa : wc;
-            const int j = u == 0 ? ja : jc;
             if (k0 + u * L >= ke) break;  // uniform over the group
+            const double4 wj = u == 0 ? wa : wb;
+            const uint32_t cwj = u == 0 ? ca : cb;
             bool match = false;
             uint32_t word = 0;
-            if (k < ke) {
-              const double d2 = dist2_ref(wi.x, wi.y, wi.z, wj.x, wj.y, wj.z, c0, c1, c2);
-              if (d2 < rl2) {
-                match = !(j == i && (sx | sy | sz) == 0 && !self_int);
-                int w_j[3];
-                unpack_w(wj.w, w_j);
-                const int ux = sx + w_i[0] - w_j[0];
-                const int uy = sy + w_i[1] - w_j[1];
-                const int uz = sz + w_i[2] - w_j[2];
-                if (match && half) {
-                  if (i != j) {
-                    match = i < j;
-                  } else {
-                    const int first = ux != 0 ? ux : (uy != 0 ? uy : uz);
-                    match = first > 0;
-                  }
-                }
-                if (match && (ux < -bias || ux >= bias || uy < -bias || uy >= bias ||
-                              uz < -bias || uz >= bias)) {
-                  shift_ovf = true;
-                  match = false;
-                }
-                word = nl_word(j - base, ux, uy, uz, jbits, sbits);
+            const double d2 = dist2_ref(wi.x, wi.y, wi.z, wj.x, wj.y, wj.z, c0, c1, c2);
+            if ((u == 0 ? va : vc) && d2 < rl2) {
+              const int jl = (int)(cwj & wc.jmask);
+              match = self_int || !(zimg && jl == il);
+              if (fast) {
+                word = (uint32_t)jl | field_add(cwj & ~wc.jmask, iw, wc.H);
+                if (HALF && match) match = half_keep(wc, word, il, jl);
+              } else {
+                checked_word(wc, jl, sx, sy, sz, w_i, wj.w, HALF, il, match, word, shift_ovf);
               }
             }
             const uint64_t bal = __ballot(match) & gmask;
             if (match) {
               const int pos = cnt + __popcll(bal & below);
-              if (pos < kmax) nbr[ell_index(t, pos, tpa, ld)] = word;
+              if (pos < kmax) nbr[rowbase + (int64_t)(pos >> lt) * stride + (pos & tmask)] = word;
             }
             cnt += __popcll(bal);
           }
@@ -879,15 +973,15 @@ __global__ __launch_bounds__(256) void k_fill_atom(
   count[t] = cnt;
 }
 
-template <int L>
+template <int L, bool HALF>
 static void launch_fill_atom(hipStream_t st, const int32_t* sys_start, const int32_t* atom_sys,
                              int64_t n, const NLWs& w, int bin_cap, const int32_t* order,
-                             double rl2, int self_int, int half, int jbits, int tpa, int kmax,
-                             int64_t ld, uint32_t* nbr, int32_t* count, int32_t* status,
+                             double rl2, int self_int, int jbits, int lt, int kmax, int64_t ld,
+                             uint32_t* nbr, int32_t* count, int32_t* status,
                              const int32_t* gate) {
-  k_fill_atom<L><<<blocks_for(n * L, 256), 256, 0, st>>>(
+  k_fill_atom<L, HALF><<<blocks_for(n * L, 256), 256, 0, st>>>(
       sys_start, atom_sys, n, w.geom, w.grid, w.meta, bin_cap, w.atom_bin, w.bin_start, order,
-      w.wr, rl2, self_int, half, jbits, tpa, kmax, ld, nbr, count, status, gate);
+      w.wr, w.cw, w.wmax, rl2, self_int, jbits, lt, kmax, ld, nbr, count, status, gate);
 }
 
 __global__ void k_emit(const uint32_t* __restrict__ nbr, const int32_t* __restrict__ count,
@@ -1053,7 +1147,7 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
                                              w.atom_bin, w.slot, gate, xref);
   TSX_CHECK_LAUNCH();
   k_scan_sys<<<n_systems, 256, 0, st>>>(w.grid, w.meta, bin_cap, sys_start, w.bin_count,
-                                        w.bin_start, gate, counter, cell_col, cell_build);
+                                        w.bin_start, gate, counter, cell_col, cell_build, w.wmax);
   TSX_CHECK_LAUNCH();
   k_scatter<<<n_systems * b_atoms, 256, 0, st>>>(sys_start, n_systems, b_atoms, w.grid, w.meta,
                                                  bin_cap, w.atom_bin, w.slot, w.bin_start,
@@ -1062,7 +1156,7 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
   k_sort_atom<<<n_systems * b_atoms, 256, 0, st>>>(pos, pos_stride, sys_start, n_systems, b_atoms,
                                                    w.geom, w.grid, w.meta, bin_cap, w.atom_bin,
                                                    w.bin_start, w.unsorted, w.bin_count, order,
-                                                   w.wr, gate);
+                                                   w.wr, gate, jbits, w.cw, w.wmax);
   TSX_CHECK_LAUNCH();
   const double rl2 = r_list * r_list;
   // Fill kernel: per-bin LDS staging pays off for dense lists of many
@@ -1071,22 +1165,34 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
   // per-atom kernel wins for sparse bins and for single large systems.
   bool per_bin = fill_mode == TSX_FILL_BIN;
   if (fill_mode == TSX_FILL_AUTO) per_bin = bins_per_rlist >= 2 && kmax >= 64 && n_systems > 1;
+  int lt = 0;
+  while ((1 << lt) < tpa) ++lt;
   if (per_bin) {
     k_active<<<1, 1024, 0, st>>>(w.grid, w.meta, bin_cap, n_systems, gate, w.act);
     TSX_CHECK_LAUNCH();
     const int g_fill = blocks_for(bin_cap, FILL_W) < 768 ? blocks_for(bin_cap, FILL_W) : 768;
-    k_fill<<<g_fill, 64 * FILL_W, 0, st>>>(sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap,
-                                           w.act, w.bin_start, order, w.wr, rl2,
-                                           self_interaction, half, jbits, tpa, kmax, ld, nbr,
-                                           count, status);
-  } else if (kmax >= 64) {
+    if (half)
+      k_fill<true><<<g_fill, 64 * FILL_W, 0, st>>>(
+          sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, w.act, w.bin_start, order, w.wr,
+          w.cw, w.wmax, rl2, self_interaction, jbits, lt, kmax, ld, nbr, count, status);
+    else
+      k_fill<false><<<g_fill, 64 * FILL_W, 0, st>>>(
+          sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, w.act, w.bin_start, order, w.wr,
+          w.cw, w.wmax, rl2, self_interaction, jbits, lt, kmax, ld, nbr, count, status);
+  } else {
     // lanes per atom from the row capacity (dense lists: more candidates per
     // column); fewer lanes leave more atoms per wave for the column arithmetic
-    launch_fill_atom<4>(st, sys_start, atom_sys, n_atoms, w, bin_cap, order, rl2,
-                        self_interaction, half, jbits, tpa, kmax, ld, nbr, count, status, gate);
-  } else {
-    launch_fill_atom<2>(st, sys_start, atom_sys, n_atoms, w, bin_cap, order, rl2,
-                        self_interaction, half, jbits, tpa, kmax, ld, nbr, count, status, gate);
+#define TSX_FILL_ATOM_LAUNCH(LL, HH)                                                       \
+  launch_fill_atom<LL, HH>(st, sys_start, atom_sys, n_atoms, w, bin_cap, order, rl2,       \
+                           self_interaction, jbits, lt, kmax, ld, nbr, count, status, gate)
+    if (kmax >= 64) {
+      if (half) TSX_FILL_ATOM_LAUNCH(4, true);
+      else TSX_FILL_ATOM_LAUNCH(4, false);
+    } else {
+      if (half) TSX_FILL_ATOM_LAUNCH(2, true);
+      else TSX_FILL_ATOM_LAUNCH(2, false);
+    }
+#undef TSX_FILL_ATOM_LAUNCH
   }
   TSX_CHECK_LAUNCH();
   return TSX_OK;
