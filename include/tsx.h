@@ -121,6 +121,36 @@ const char* tsx_last_error(void);
 #define TSX_FILL_AUTO 0
 #define TSX_FILL_BIN 1
 #define TSX_FILL_ATOM 2
+
+/* ``word_mode``: TSX_WORDS_SHIFT writes the image shift S' in the three word
+ * fields (the layout above, read by every consumer of the reference triple);
+ * TSX_WORDS_STENCIL writes instead the stencil offset d of the neighbour's
+ * bin relative to the centre's bin (|d_k| <= m_k), for the fused MD step,
+ * which then needs a bin-relative frame (full lists only).  For an entry
+ * (i, j, d) with anchors A (below) the displacement is
+ *     r_j - r_i + S'.L = (r_j - A_j) - (r_i - A_i) + sum_k d_k ustep_k,
+ * every term of which is of the order of a bin, so it is accurate in FP32. */
+#define TSX_WORDS_SHIFT 0
+#define TSX_WORDS_STENCIL 1
+
+/* Bin-relative frame written by a build (any pointer may be NULL):
+ *   anchor [N,4] float64, by atom index: A_i = sum_k (w_ik + b_ik / nb_k) a_k,
+ *       the corner of atom i's bin in its unwrapped image (w = wrap shift,
+ *       b = bin, a_k = lattice vector k); along a non-periodic direction the
+ *       coefficient is the fractional coordinate of the system's first atom,
+ *       and a system with no periodic axis (or a singular cell) anchors at
+ *       its first atom;
+ *   rel [N,4] float32, by atom index: r_i - A_i rounded to float, and the low
+ *       32 bits of the packed species word of a stride-4 position;
+ *   ustep [S,12] float64: entries 3k..3k+2 = a_k / nb_k, entry 9 =
+ *       sum_k |a_k / nb_k| (written by setup builds only).
+ * Anchor and rel are written for the rebuilt systems. */
+typedef struct tsx_nl_frame {
+  double* anchor;
+  float* rel;
+  double* ustep;
+} tsx_nl_frame;
+
 int64_t tsx_nl_ws_bytes(int64_t n_atoms, int32_t n_systems, int32_t bin_cap);
 
 int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
@@ -130,7 +160,8 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
                  int32_t half, int32_t jbits, int32_t tpa, int32_t kmax, int64_t ld,
                  int32_t bin_cap, int32_t setup, void* ws, uint32_t* nbr, int32_t* count,
                  int32_t* order, double* xref, int32_t* status, const int32_t* gate,
-                 int32_t* counter, double* cell_build, int32_t fill_mode, void* stream);
+                 int32_t* counter, double* cell_build, int32_t fill_mode, int32_t word_mode,
+                 const tsx_nl_frame* frame, void* stream);
 
 /* Exclusive scan of ``count`` into ``row_ptr`` [n+1] (int64); the total is
  * row_ptr[n].  Used before tsx_nl_emit_pairs.  ``ws`` holds
@@ -236,7 +267,12 @@ int tsx_md_baoa(double* x, double* p, const double* f, const double* m, const in
  * pos_out = pos_in.  An atom of system s farther than ``half_skin`` from its
  * ``xref`` sets flag_set[s] = 1 (the rebuild gate of the next step).
  * ``step`` is a device counter giving the Philox step of the next step's
- * noise.  w_atom may be NULL. */
+ * noise.  w_atom may be NULL.
+ * The list must be a TSX_WORDS_STENCIL list with its frame (anchor, ustep and
+ * rel_in = the rel of pos_in): each neighbour costs one 16-byte gather of
+ * rel; the FP32 displacement decides the cutoff outside a proven rounding
+ * margin and the FP64 displacement (r_j - A_j) - (r_i - A_i) + d.ustep
+ * decides inside it.  rel_out receives the rel of pos_out. */
 int tsx_md_fused_step(int32_t kind, const double* params, int32_t n_species,
                       const double* pos_in, double* pos_out, const double* xref,
                       const double* cell_col, const int32_t* sys_start,
@@ -246,7 +282,8 @@ int tsx_md_fused_step(int32_t kind, const double* params, int32_t n_species,
                       double* e_atom, double* w_atom, double dt, int32_t langevin,
                       const double* kT, const double* gamma, const double* noise,
                       uint64_t seed, const int64_t* step, int32_t end_kick, int32_t do_next,
-                      double half_skin, int32_t* flag_set, void* stream);
+                      double half_skin, int32_t* flag_set, const float* rel_in, float* rel_out,
+                      const double* anchor, const double* ustep, void* stream);
 
 /* Per-system energy [S] = sum of e_atom (the chunked fixed-order reduction
  * of tsx_system_reduce, same workspace); gate_clear[s] = 0 when given (the

@@ -33,7 +33,8 @@ _SIGS = {
     "tsx_nl_ws_bytes": (L, [L, I, I]),
     "tsx_nl_build": (
         I,
-        [P, I, P, P, I, P, P, L, I, D, I, I, I, I, I, I, L, I, I, P, P, P, P, P, P, P, P, P, I, P],
+        [P, I, P, P, I, P, P, L, I, D, I, I, I, I, I, I, L, I, I, P, P, P, P, P, P, P, P, P, I, I,
+         P, P],
     ),
     "tsx_nl_check_ws_bytes": (L, [I]),
     "tsx_nl_check": (I, [P, I, P, P, P, P, L, I, D, D, P, P, P]),
@@ -56,7 +57,8 @@ _SIGS = {
         I,
         [
             I, P, I, P, P, P, P, P, P, P, P, P, L, I, I, L, D,  # pair part
-            P, P, P, P, P, D, I, P, P, P, U, P, I, I, D, P, P,  # step part
+            P, P, P, P, P, D, I, P, P, P, U, P, I, I, D, P,  # step part
+            P, P, P, P, P,  # frame: rel_in, rel_out, anchor, ustep; stream
         ],
     ),
     "tsx_md_reduce": (I, [P, P, I, I, P, P, I, P, P, P]),
@@ -97,6 +99,17 @@ class FireParams(ctypes.Structure):
         ("constant_volume", ctypes.c_int32),
         ("first_step", ctypes.c_int32),
     ]
+
+
+class NLFrame(ctypes.Structure):
+    """tsx_nl_frame (include/tsx.h): bin-relative frame of a stencil-word list."""
+
+    _fields_ = [
+        ("anchor", ctypes.c_void_p),
+        ("rel", ctypes.c_void_p),
+        ("ustep", ctypes.c_void_p),
+    ]
+
 
 _lib = None
 

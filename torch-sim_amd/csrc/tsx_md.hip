@@ -84,16 +84,15 @@ struct FusedArgs {
   const double* pos_in;  // [N,4] (x, y, z, species bits)
   double* pos_out;       // [N,4]
   const double* xref;    // [N,4] positions at the last list build
-  const double* cell_col;
   const int32_t* sys_start;
   const int32_t* atom_sys;
-  const uint32_t* nbr;
+  const uint32_t* nbr;   // stencil-word list
   const int32_t* count;
   const int32_t* order;
   int64_t n;
   int jbits;
   int64_t ld;
-  double rc2;
+  double rc;
   double* p;       // [N,3]
   const double* m; // [N]
   double* f;       // [N,3]
@@ -107,11 +106,129 @@ struct FusedArgs {
   const int64_t* step;  // device counter: reference step whose first half runs here
   int end_kick;
   int do_next;
-  double half_skin2;
-  int32_t* flag_set;  // [S] rebuild request of each system for the next step
+  double half_skin;
+  int32_t* flag_set;    // [S] rebuild request of each system for the next step
+  const float* rel_in;  // [N,4] r - A of pos_in
+  float* rel_out;       // [N,4] r - A of pos_out
+  const double* anchor; // [N,4] bin anchors A
+  const double* ustep;  // [S,12] a_k / nb_k (rows), sum_k |a_k / nb_k|
 };
 
-template <int KIND, int TPA, bool LANGEVIN, bool VIRIAL>
+// The FP64 side of the cutoff decision for entries within the rounding
+// margin: (r_j - A_j) - (r_i - A_i) + d . ustep.
+struct ExactCtx {
+  const double4* pos_s;  // this system's packed positions
+  const double4* anc_s;  // this system's anchors
+  double xi, yi, zi;     // r_i - A_i
+  const double* u;       // ustep row of the system
+  double rc2;
+};
+
+// Walk this lane's share of row t with stencil words: one 16-byte gather of
+// the neighbour's bin-relative position per entry and the FP32 displacement
+// d = (rel_j - rel_i) + d_x u_0 + d_y u_1 + d_z u_2.  Entries with
+// d^2 >= hi2 are out, d^2 < lo2 in; between the two (the proven FP32
+// rounding margin around rc) the FP64 displacement decides, so the pair set
+// is the FP64 one (pair_potential.py:181-194).  The pair function is FP32 on
+// d, accumulation FP64 (pair_potential.py:419-430).  UNR entries at a time:
+// all words, then all gathers in flight.
+template <bool VIRIAL>
+__device__ __forceinline__ void add_pair(RowAcc& a, float e, float fs, float dx, float dy,
+                                         float dz) {
+  const double f = (double)fs;
+  const double ddx = (double)dx, ddy = (double)dy, ddz = (double)dz;
+  a.fx = fma(-f, ddx, a.fx);
+  a.fy = fma(-f, ddy, a.fy);
+  a.fz = fma(-f, ddz, a.fz);
+  a.e = fma(0.5, (double)e, a.e);
+  if (VIRIAL) {
+    const double hf = 0.5 * f;
+    a.w[0] += hf * ddx * ddx;
+    a.w[1] += hf * ddy * ddy;
+    a.w[2] += hf * ddz * ddz;
+    a.w[3] += hf * ddy * ddz;
+    a.w[4] += hf * ddx * ddz;
+    a.w[5] += hf * ddx * ddy;
+  }
+}
+
+template <int KIND, int TPA, bool VIRIAL, bool MULTI>
+__device__ __forceinline__ RowAcc accumulate_rel(const PairP* sp, int ns, int ti,
+                                                 const float4* __restrict__ rel_s, float xri,
+                                                 float yri, float zri, const float* uf,
+                                                 const uint32_t* __restrict__ nbr, int64_t t,
+                                                 int cnt, int lane, int jbits, int64_t ld,
+                                                 float lo2, float hi2, const ExactCtx& ex) {
+  RowAcc a;
+  a.fx = a.fy = a.fz = a.e = 0.0;
+  for (int c = 0; c < 6; ++c) a.w[c] = 0.0;
+  const int sbits = (32 - jbits) / 3;
+  const uint32_t jmask = (1u << jbits) - 1u;
+  const PairP p0 = sp[MULTI ? 0 : ti * ns];
+  const uint32_t* wp = nbr + t * TPA + lane;
+  const int64_t stride = ld * TPA;
+  const int iters = cnt > lane ? (cnt - lane + TPA - 1) / TPA : 0;
+  for (int k0 = 0; k0 < iters; k0 += UNR) {
+    uint32_t w[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) w[u] = (k0 + u < iters) ? wp[(int64_t)(k0 + u) * stride] : 0u;
+    float4 q[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) q[u] = rel_s[w[u] & jmask];
+    uint32_t amb = 0;  // entries within the rounding margin of rc
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      if (k0 + u >= iters) break;
+      const float fx = (float)nl_shift(w[u], jbits, sbits, 0);
+      const float fy = (float)nl_shift(w[u], jbits, sbits, 1);
+      const float fz = (float)nl_shift(w[u], jbits, sbits, 2);
+      const float dx = (q[u].x - xri) + fmaf(fz, uf[6], fmaf(fy, uf[3], fx * uf[0]));
+      const float dy = (q[u].y - yri) + fmaf(fz, uf[7], fmaf(fy, uf[4], fx * uf[1]));
+      const float dz = (q[u].z - zri) + fmaf(fz, uf[8], fmaf(fy, uf[5], fx * uf[2]));
+      const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+      if (!(d2 < hi2)) continue;
+      if (!(d2 < lo2)) {
+        amb |= 1u << u;
+        continue;
+      }
+      float e, fs;
+      pair_eval<KIND>(d2, MULTI ? sp[ti * ns + __float_as_int(q[u].w)] : p0, e, fs);
+      add_pair<VIRIAL>(a, e, fs, dx, dy, dz);
+    }
+    while (amb) {  // rare: the FP64 displacement decides
+      const int u = __builtin_ctz(amb);
+      amb &= amb - 1u;
+      const uint32_t wu = u == 0 ? w[0] : (u == 1 ? w[1] : (u == 2 ? w[2] : w[UNR - 1]));
+      const uint32_t jl = wu & jmask;
+      const double ix = (double)nl_shift(wu, jbits, sbits, 0);
+      const double iy = (double)nl_shift(wu, jbits, sbits, 1);
+      const double iz = (double)nl_shift(wu, jbits, sbits, 2);
+      const double4 pj = ex.pos_s[jl];
+      const double4 aj = ex.anc_s[jl];
+      const double gx = ((pj.x - aj.x) - ex.xi) + fma(iz, ex.u[6], fma(iy, ex.u[3], ix * ex.u[0]));
+      const double gy = ((pj.y - aj.y) - ex.yi) + fma(iz, ex.u[7], fma(iy, ex.u[4], ix * ex.u[1]));
+      const double gz = ((pj.z - aj.z) - ex.zi) + fma(iz, ex.u[8], fma(iy, ex.u[5], ix * ex.u[2]));
+      const double d2e = gx * gx + gy * gy + gz * gz;
+      if (!(d2e < ex.rc2)) continue;
+      float e, fs;
+      pair_eval<KIND>((float)d2e,
+                      MULTI ? sp[ti * ns + (int)__double_as_longlong(pj.w)] : p0, e, fs);
+      add_pair<VIRIAL>(a, e, fs, (float)gx, (float)gy, (float)gz);
+    }
+  }
+#pragma unroll
+  for (int off = TPA / 2; off > 0; off >>= 1) {
+    a.fx += __shfl_xor(a.fx, off, 64);
+    a.fy += __shfl_xor(a.fy, off, 64);
+    a.fz += __shfl_xor(a.fz, off, 64);
+    a.e += __shfl_xor(a.e, off, 64);
+    if (VIRIAL)
+      for (int c = 0; c < 6; ++c) a.w[c] += __shfl_xor(a.w[c], off, 64);
+  }
+  return a;
+}
+
+template <int KIND, int TPA, bool LANGEVIN, bool VIRIAL, bool MULTI>
 __global__ __launch_bounds__(256) void k_md_fused(FusedArgs A) {
   __shared__ PairP sp[MAX_SPECIES_MD * MAX_SPECIES_MD];
   for (int k = threadIdx.x; k < A.ns * A.ns; k += blockDim.x)
@@ -124,12 +241,41 @@ __global__ __launch_bounds__(256) void k_md_fused(FusedArgs A) {
   const int64_t tt = valid ? t : 0;
   const int i = A.order[tt];
   const int s = A.atom_sys[i];
-  const double4 xi = reinterpret_cast<const double4*>(A.pos_in)[i];
-  const int ti = A.ns > 1 ? (int)__double_as_longlong(xi.w) : 0;
-  const RowAcc a = accumulate_row<KIND, TPA, VIRIAL, 4>(
-      sp, A.ns, nullptr, A.pos_in, A.cell_col + 9 * (int64_t)s, A.sys_start[s], xi.x, xi.y, xi.z,
-      ti, A.nbr, tt, valid ? A.count[tt] : 0, lane, A.jbits, A.ld, A.rc2);
+  const int base = A.sys_start[s];
+  int ti;
+  ExactCtx ex;
+  {
+    const double4 x0 = reinterpret_cast<const double4*>(A.pos_in)[i];
+    const double4 a0 = reinterpret_cast<const double4*>(A.anchor)[i];
+    ti = MULTI ? (int)__double_as_longlong(x0.w) : 0;
+    ex.xi = x0.x - a0.x;
+    ex.yi = x0.y - a0.y;
+    ex.zi = x0.z - a0.z;
+  }
+  // the system's bin steps and the FP32 rounding margin around rc: every
+  // term of the FP32 displacement is bounded by M = rc + 2 skin + 3 sum|u_k|
+  // and the displacement carries fewer than 32 roundings of 2^-24 M
+  const double* us = A.ustep + 12 * (int64_t)s;
+  float uf[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) uf[k] = (float)us[k];
+  const double big = A.rc + 4.0 * A.half_skin + 3.0 * us[9];
+  const double tol = 64.0 * 5.9604644775390625e-8 * big;  // 64 * 2^-24 * M
+  const float lo2 = (float)((A.rc - tol) * (A.rc - tol) * (1.0 - 1.0 / 1048576.0));
+  const float hi2 = (float)((A.rc + tol) * (A.rc + tol) * (1.0 + 1.0 / 1048576.0));
+  ex.pos_s = reinterpret_cast<const double4*>(A.pos_in) + base;
+  ex.anc_s = reinterpret_cast<const double4*>(A.anchor) + base;
+  ex.u = us;
+  ex.rc2 = A.rc * A.rc;
+  const float4* rel_s = reinterpret_cast<const float4*>(A.rel_in) + base;
+  const int cnt = valid ? A.count[tt] : 0;
+  const RowAcc a = accumulate_rel<KIND, TPA, VIRIAL, MULTI>(
+      sp, A.ns, ti, rel_s, (float)ex.xi, (float)ex.yi, (float)ex.zi, uf, A.nbr, tt, cnt, lane,
+      A.jbits, A.ld, lo2, hi2, ex);
   if (!valid || lane != 0) return;
+  // own position and anchor again (cached): not kept live across the row walk
+  const double4 xi = reinterpret_cast<const double4*>(A.pos_in)[i];
+  const double4 ai = reinterpret_cast<const double4*>(A.anchor)[i];
   const double hdt = 0.5 * A.dt;
   const double mi = A.m[i];
   double p0 = A.p[3 * (int64_t)i], p1 = A.p[3 * (int64_t)i + 1], p2 = A.p[3 * (int64_t)i + 2];
@@ -165,11 +311,17 @@ __global__ __launch_bounds__(256) void k_md_fused(FusedArgs A) {
     }
     if (A.xref && A.flag_set) {
       const double4 r = reinterpret_cast<const double4*>(A.xref)[i];
-      const double ex = xo.x - r.x, ey = xo.y - r.y, ez = xo.z - r.z;
-      if (ex * ex + ey * ey + ez * ez > A.half_skin2) A.flag_set[s] = 1;
+      const double gx = xo.x - r.x, gy = xo.y - r.y, gz = xo.z - r.z;
+      if (gx * gx + gy * gy + gz * gz > A.half_skin * A.half_skin) A.flag_set[s] = 1;
     }
   }
   reinterpret_cast<double4*>(A.pos_out)[i] = xo;
+  float4 ro;
+  ro.x = (float)(xo.x - ai.x);
+  ro.y = (float)(xo.y - ai.y);
+  ro.z = (float)(xo.z - ai.z);
+  ro.w = __int_as_float((int)__double_as_longlong(xo.w));
+  reinterpret_cast<float4*>(A.rel_out)[i] = ro;
   A.p[3 * (int64_t)i] = p0;
   A.p[3 * (int64_t)i + 1] = p1;
   A.p[3 * (int64_t)i + 2] = p2;
@@ -195,10 +347,15 @@ __global__ void k_copy_ref(const double* __restrict__ pos, double* __restrict__ 
 
 template <int KIND, int TPA, bool LANGEVIN>
 static void launch_fused_v(bool vir, hipStream_t st, int nblk, const FusedArgs& a) {
-  if (vir)
-    k_md_fused<KIND, TPA, LANGEVIN, true><<<nblk, 256, 0, st>>>(a);
+  const bool multi = a.ns > 1;
+  if (vir && multi)
+    k_md_fused<KIND, TPA, LANGEVIN, true, true><<<nblk, 256, 0, st>>>(a);
+  else if (vir)
+    k_md_fused<KIND, TPA, LANGEVIN, true, false><<<nblk, 256, 0, st>>>(a);
+  else if (multi)
+    k_md_fused<KIND, TPA, LANGEVIN, false, true><<<nblk, 256, 0, st>>>(a);
   else
-    k_md_fused<KIND, TPA, LANGEVIN, false><<<nblk, 256, 0, st>>>(a);
+    k_md_fused<KIND, TPA, LANGEVIN, false, false><<<nblk, 256, 0, st>>>(a);
 }
 
 template <int KIND, int TPA>
@@ -271,12 +428,17 @@ int tsx_md_fused_step(int32_t kind, const double* params, int32_t n_species,
                       double* e_atom, double* w_atom, double dt, int32_t langevin,
                       const double* kT, const double* gamma, const double* noise,
                       uint64_t seed, const int64_t* step, int32_t end_kick, int32_t do_next,
-                      double half_skin, int32_t* flag_set, void* stream) {
+                      double half_skin, int32_t* flag_set, const float* rel_in, float* rel_out,
+                      const double* anchor, const double* ustep, void* stream) {
+  (void)cell_col;  // the frame (anchor, ustep) carries the geometry
   TSX_REQUIRE(kind >= TSX_LJ && kind <= TSX_SOFT_SPHERE, "unknown pair kind %d", kind);
   TSX_REQUIRE(n_species >= 1 && n_species <= MAX_SPECIES_MD, "n_species out of range");
   TSX_REQUIRE(pos_in != pos_out, "pos_in and pos_out must be different buffers");
   TSX_REQUIRE(!langevin || (kT && gamma && step), "Langevin needs kT, gamma and step");
-  TSX_REQUIRE(jbits >= 1 && jbits <= 26, "jbits must be in [1, 26]");
+  TSX_REQUIRE(jbits >= 1 && jbits <= 22, "jbits must be in [1, 22]");
+  TSX_REQUIRE(rel_in && rel_out && rel_in != rel_out && anchor && ustep,
+              "the fused step needs the list frame: rel_in, rel_out, anchor, ustep");
+  TSX_REQUIRE(cutoff > 0.0 && half_skin >= 0.0, "need cutoff > 0 and half_skin >= 0");
   if (n_atoms == 0) return TSX_OK;
   FusedArgs a;
   a.params = params;
@@ -284,7 +446,6 @@ int tsx_md_fused_step(int32_t kind, const double* params, int32_t n_species,
   a.pos_in = pos_in;
   a.pos_out = pos_out;
   a.xref = xref;
-  a.cell_col = cell_col;
   a.sys_start = sys_start;
   a.atom_sys = atom_sys;
   a.nbr = nbr;
@@ -293,7 +454,7 @@ int tsx_md_fused_step(int32_t kind, const double* params, int32_t n_species,
   a.n = n_atoms;
   a.jbits = jbits;
   a.ld = ld;
-  a.rc2 = cutoff * cutoff;
+  a.rc = cutoff;
   a.p = p;
   a.m = m;
   a.f = f;
@@ -307,8 +468,12 @@ int tsx_md_fused_step(int32_t kind, const double* params, int32_t n_species,
   a.step = step;
   a.end_kick = end_kick;
   a.do_next = do_next;
-  a.half_skin2 = half_skin * half_skin;
+  a.half_skin = half_skin;
   a.flag_set = flag_set;
+  a.rel_in = rel_in;
+  a.rel_out = rel_out;
+  a.anchor = anchor;
+  a.ustep = ustep;
   hipStream_t st = (hipStream_t)stream;
   const int nblk = blocks_for(n_atoms * tpa, 256);
   const bool vir = w_atom != nullptr;

@@ -139,7 +139,8 @@ __global__ void k_setup(const double* __restrict__ cell_col, const uint8_t* __re
                         int zero_bins,
                         double* __restrict__ geom, int32_t* __restrict__ grid,
                         int32_t* __restrict__ meta, int32_t* __restrict__ status,
-                        int32_t* __restrict__ bin_count, const int32_t* __restrict__ gate) {
+                        int32_t* __restrict__ bin_count, const int32_t* __restrict__ gate,
+                        double* __restrict__ ustep) {
   if (gate && *gate == 0) return;
   // every block zeroes part of the bin counters [0, bin_cap]
   if (zero_bins)
@@ -203,6 +204,20 @@ __global__ void k_setup(const double* __restrict__ cell_col, const uint8_t* __re
       }
       gr[7] = flags;
       nbins = nb[0] * nb[1] * nb[2];
+      if (ustep) {  // bin step along each lattice vector a_k / nb_k, and sum_k |a_k / nb_k|
+        double* u = ustep + 12 * (size_t)s;
+        double su = 0.0;
+        for (int k = 0; k < 3; ++k) {
+          double n2 = 0.0;
+          for (int c = 0; c < 3; ++c) {
+            u[3 * k + c] = L[3 * k + c] / nb[k];
+            n2 += u[3 * k + c] * u[3 * k + c];
+          }
+          su += sqrt(n2);
+        }
+        u[9] = su;
+        u[10] = u[11] = 0.0;
+      }
     }
     // block exclusive scan of nbins
     sh_nb[threadIdx.x] = nbins;
@@ -511,9 +526,11 @@ __global__ __launch_bounds__(256) void k_scatter(
 // bin (by atom index, so the order is unique and does not depend on the
 // scatter's atomic slots), its place order[bin_start + rank], its wrapped
 // coordinates at that place, its partial list word cw (system-local index in
-// the low jbits, negated wrap shift in the three shift fields: a neighbour's
-// word is this plus the image shift and the centre's wrap shift, field by
-// field), the largest wrap magnitude of the system, and the bin counter
+// the low jbits and above it, for image-shift words, the negated wrap shift
+// in the three fields -- a neighbour's word is this plus the image shift and
+// the centre's wrap shift, field by field -- or, for stencil words, the
+// atom's bin z index), the largest wrap magnitude of the system, the
+// bin-relative frame (anchor and rel) when asked for, and the bin counter
 // reset for the next build.  Single-bin systems were scattered in atom order
 // already.
 __global__ __launch_bounds__(256) void k_sort_atom(
@@ -523,7 +540,8 @@ __global__ __launch_bounds__(256) void k_sort_atom(
     const int32_t* __restrict__ bin_start, const int32_t* __restrict__ unsorted,
     int32_t* __restrict__ bin_count, int32_t* __restrict__ order, double* __restrict__ wr,
     const int32_t* __restrict__ gate, int jbits, uint32_t* __restrict__ cw,
-    int32_t* __restrict__ wmax) {
+    int32_t* __restrict__ wmax, int stencil, double* __restrict__ anchor,
+    float* __restrict__ rel) {
   const int s = blockIdx.x / B, part = blockIdx.x % B;
   if (s >= n_sys || meta[0] > bin_cap || gated_off(gate, s)) return;
   const int32_t* gr = grid + (size_t)GRID * s;
@@ -551,8 +569,46 @@ __global__ __launch_bounds__(256) void k_sort_atom(
     v.z = o.wr[2];
     v.w = pack_w(o.w[0], o.w[1], o.w[2]);
     reinterpret_cast<double4*>(wr)[dst] = v;
+    const int lb = b - gr[6];
+    const int bz = lb % gr[2], by = (lb / gr[2]) % gr[1], bx = lb / (gr[1] * gr[2]);
     cw[dst] = (uint32_t)(i - sys_start[s]) |
-              (pack_fields(-o.w[0], -o.w[1], -o.w[2], sbits) << jbits);
+              ((stencil ? (uint32_t)bz : pack_fields(-o.w[0], -o.w[1], -o.w[2], sbits)) << jbits);
+    if (anchor || rel) {
+      const double* r = pos + (int64_t)ps * i;
+      double A[3];
+      if (g[22] == 0.0) {  // no lattice: anchor at the system's first atom
+        const double* r0 = pos + (int64_t)ps * sys_start[s];
+        for (int m = 0; m < 3; ++m) A[m] = r0[m];
+      } else {
+        const int bb[3] = {bx, by, bz};
+        double c[3];
+        for (int k = 0; k < 3; ++k) {
+          if (gr[7] & (1 << k)) {
+            c[k] = (double)o.w[k] + (double)bb[k] / (double)gr[k];
+          } else {  // non-periodic direction: the first atom's coordinate
+            const double* r0 = pos + (int64_t)ps * sys_start[s];
+            c[k] = r0[0] * g[9 + k] + r0[1] * g[12 + k] + r0[2] * g[15 + k];
+          }
+        }
+        for (int m = 0; m < 3; ++m) A[m] = c[0] * g[m] + c[1] * g[3 + m] + c[2] * g[6 + m];
+      }
+      if (anchor) {
+        double4 av;
+        av.x = A[0];
+        av.y = A[1];
+        av.z = A[2];
+        av.w = 0.0;
+        reinterpret_cast<double4*>(anchor)[i] = av;
+      }
+      if (rel) {
+        float4 rv;
+        rv.x = (float)(r[0] - A[0]);
+        rv.y = (float)(r[1] - A[1]);
+        rv.z = (float)(r[2] - A[2]);
+        rv.w = ps == 4 ? __int_as_float((int)__double_as_longlong(r[3])) : 0.f;
+        reinterpret_cast<float4*>(rel)[i] = rv;
+      }
+    }
     wm = max(wm, max(abs(o.w[0]), max(abs(o.w[1]), abs(o.w[2]))));
     bin_count[b] = 0;
   }
@@ -599,6 +655,11 @@ __device__ __forceinline__ int stencil_smax(const int32_t* gr) {
     r = max(r, max(lo, hi));
   }
   return r;
+}
+
+// Stencil offsets |d_k| <= m_k fit the word fields.
+__device__ __forceinline__ bool stencil_fits(const int32_t* gr, int bias) {
+  return gr[3] < bias && gr[4] < bias && gr[5] < bias;
 }
 
 // Per-system constants of the list word.  A system is "fast" when its image
@@ -684,7 +745,7 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
     const int32_t* __restrict__ order, const double* __restrict__ wr,
     const uint32_t* __restrict__ cw, const int32_t* __restrict__ wmax, double rl2, int self_int,
     int jbits, int lt, int kmax, int64_t ld, uint32_t* __restrict__ nbr,
-    int32_t* __restrict__ count, int32_t* __restrict__ status) {
+    int32_t* __restrict__ count, int32_t* __restrict__ status, int stencil) {
   __shared__ double4 cpos_all[FILL_W][FILL_CH];
   __shared__ uint32_t ccw_all[FILL_W][FILL_CH];
   __shared__ int32_t cimg_all[FILL_W][FILL_CH];
@@ -719,7 +780,12 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
     const int c0 = bin_start[b], c1 = bin_start[b + 1];
     const int nc = c1 - c0;
     if (nc == 0) continue;
-    const bool fast = stencil_smax(gr) + 2 * wmax[s] < wc.bias;
+    const bool fast = stencil ? stencil_fits(gr, wc.bias)
+                              : stencil_smax(gr) + 2 * wmax[s] < wc.bias;
+    if (stencil && !fast) {  // stencil offsets wider than the word fields
+      if (tid == 0) atomicOr(status, TSX_STATUS_SHIFT_OVERFLOW);
+      continue;
+    }
     const int local = b - off;
     const int bz0 = local % gr[2], by0 = (local / gr[2]) % gr[1], bx0 = local / (gr[1] * gr[2]);
     int per = 1;  // centre atoms per round (power of two)
@@ -765,7 +831,8 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
           isv[q][0] = sx;
           isv[q][1] = sy;
           isv[q][2] = sz;
-          sw[q] = pack_fields(sx, sy, sz, wc.sbits) << jbits;
+          sw[q] = (stencil ? pack_fields(dx, dy, dz, wc.sbits) : pack_fields(sx, sy, sz, wc.sbits))
+                  << jbits;
           const int kb = bin_start[nbin], ke = bin_start[nbin + 1];
           ibeg[q] = kb;
           ipre[q + 1] = ke - kb;
@@ -816,7 +883,9 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
                 const int jl = (int)(cwj & wc.jmask);
                 if (fast) {
                   const uint32_t swq = sw[q];
-                  word = (uint32_t)jl | field_add(cwj & ~wc.jmask, field_add(swq, wiw, wc.H), wc.H);
+                  word = stencil ? (uint32_t)jl | swq
+                                 : (uint32_t)jl |
+                                       field_add(cwj & ~wc.jmask, field_add(swq, wiw, wc.H), wc.H);
                   match = self_int || !(swq == 0 && jl == il);
                   if (HALF && match) match = half_keep(wc, word, il, jl);
                 } else {
@@ -868,7 +937,7 @@ __global__ __launch_bounds__(256) void k_fill_atom(
     const double* __restrict__ wr, const uint32_t* __restrict__ cw,
     const int32_t* __restrict__ wmax, double rl2, int self_int, int jbits, int lt, int kmax,
     int64_t ld, uint32_t* __restrict__ nbr, int32_t* __restrict__ count,
-    int32_t* __restrict__ status, const int32_t* __restrict__ gate) {
+    int32_t* __restrict__ status, const int32_t* __restrict__ gate, int stencil) {
   const int64_t gid = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   const int64_t t = gid / L;
   const int sub = (int)(gid % L);
@@ -890,7 +959,12 @@ __global__ __launch_bounds__(256) void k_fill_atom(
   const int local = atom_bin[i] - off;
   const int bz0 = local % nbz, by0 = (local / nbz) % nby, bx0 = local / (nby * nbz);
   const WordCtx wc = word_ctx(jbits);
-  const bool fast = stencil_smax(gr) + 2 * wmax[s] < wc.bias;
+  const bool fast =
+      stencil ? stencil_fits(gr, wc.bias) : stencil_smax(gr) + 2 * wmax[s] < wc.bias;
+  if (stencil && !fast) {  // stencil offsets wider than the word fields
+    if (sub == 0) atomicOr(status, TSX_STATUS_SHIFT_OVERFLOW);
+    return;
+  }
   const double4 wi = reinterpret_cast<const double4*>(wr)[t];
   int w_i[3];
   unpack_w(wi.w, w_i);
@@ -925,6 +999,8 @@ __global__ __launch_bounds__(256) void k_fill_atom(
         const double c2 = shift_comp(sx, sy, sz, g, 2);
         const uint32_t iw = field_add(pack_fields(sx, sy, sz, wc.sbits) << jbits, wiw, wc.H);
         const bool zimg = (sx | sy | sz) == 0;
+        // stencil words: dz of a candidate is its bin's z index plus zoff
+        const int zoff = sz * nbz - bz0;
         // two candidates per lane in flight; out-of-range lanes re-read the
         // run's first candidate, so every load is unconditional
         for (int k0 = kb; k0 < ke; k0 += 2 * L) {
@@ -945,7 +1021,10 @@ __global__ __launch_bounds__(256) void k_fill_atom(
             if ((u == 0 ? va : vc) && d2 < rl2) {
               const int jl = (int)(cwj & wc.jmask);
               match = self_int || !(zimg && jl == il);
-              if (fast) {
+              if (stencil) {
+                word = (uint32_t)jl |
+                       (pack_fields(dx, dy, (int)(cwj >> jbits) + zoff, wc.sbits) << jbits);
+              } else if (fast) {
                 word = (uint32_t)jl | field_add(cwj & ~wc.jmask, iw, wc.H);
                 if (HALF && match) match = half_keep(wc, word, il, jl);
               } else {
@@ -978,10 +1057,10 @@ static void launch_fill_atom(hipStream_t st, const int32_t* sys_start, const int
                              int64_t n, const NLWs& w, int bin_cap, const int32_t* order,
                              double rl2, int self_int, int jbits, int lt, int kmax, int64_t ld,
                              uint32_t* nbr, int32_t* count, int32_t* status,
-                             const int32_t* gate) {
+                             const int32_t* gate, int stencil) {
   k_fill_atom<L, HALF><<<blocks_for(n * L, 256), 256, 0, st>>>(
       sys_start, atom_sys, n, w.geom, w.grid, w.meta, bin_cap, w.atom_bin, w.bin_start, order,
-      w.wr, w.cw, w.wmax, rl2, self_int, jbits, lt, kmax, ld, nbr, count, status, gate);
+      w.wr, w.cw, w.wmax, rl2, self_int, jbits, lt, kmax, ld, nbr, count, status, gate, stencil);
 }
 
 __global__ void k_emit(const uint32_t* __restrict__ nbr, const int32_t* __restrict__ count,
@@ -1115,7 +1194,7 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
                  int32_t tpa, int32_t kmax, int64_t ld, int32_t bin_cap, int32_t setup, void* ws,
                  uint32_t* nbr, int32_t* count, int32_t* order, double* xref, int32_t* status,
                  const int32_t* gate, int32_t* counter, double* cell_build, int32_t fill_mode,
-                 void* stream) {
+                 int32_t word_mode, const tsx_nl_frame* frame, void* stream) {
   TSX_REQUIRE(n_systems > 0, "n_systems must be positive");
   TSX_REQUIRE(n_atoms >= 0 && n_atoms < (1ll << 31), "n_atoms out of range");
   TSX_REQUIRE(pos_stride == 3 || pos_stride == 4, "pos_stride must be 3 or 4");
@@ -1128,6 +1207,10 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
   TSX_REQUIRE(bins_per_rlist >= 1 && bins_per_rlist <= 4, "bins_per_rlist in [1,4]");
   TSX_REQUIRE(bin_cap >= 1, "bin_cap must be >= 1");
   TSX_REQUIRE(fill_mode >= TSX_FILL_AUTO && fill_mode <= TSX_FILL_ATOM, "unknown fill_mode");
+  TSX_REQUIRE(word_mode == TSX_WORDS_SHIFT || word_mode == TSX_WORDS_STENCIL,
+              "unknown word_mode");
+  TSX_REQUIRE(word_mode == TSX_WORDS_SHIFT || (!half && jbits <= 22),
+              "stencil words need a full list and jbits <= 22");
   hipStream_t st = (hipStream_t)stream;
   NLWs w;
   ws_layout(n_atoms, n_systems, bin_cap, &w, (char*)ws);
@@ -1135,7 +1218,7 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
     const int zb = blocks_for((int64_t)bin_cap + 1, 1024);
     k_setup<<<zb < 1024 ? zb : 1024, 1024, 0, st>>>(
         cell_col, pbc, pbc_per_system, n_systems, r_list, bins_per_rlist, bin_cap, 1, w.geom,
-        w.grid, w.meta, status, w.bin_count, nullptr);
+        w.grid, w.meta, status, w.bin_count, nullptr, frame ? frame->ustep : nullptr);
     TSX_CHECK_LAUNCH();
   }
   if (n_atoms == 0) return TSX_OK;
@@ -1156,7 +1239,9 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
   k_sort_atom<<<n_systems * b_atoms, 256, 0, st>>>(pos, pos_stride, sys_start, n_systems, b_atoms,
                                                    w.geom, w.grid, w.meta, bin_cap, w.atom_bin,
                                                    w.bin_start, w.unsorted, w.bin_count, order,
-                                                   w.wr, gate, jbits, w.cw, w.wmax);
+                                                   w.wr, gate, jbits, w.cw, w.wmax, word_mode,
+                                                   frame ? frame->anchor : nullptr,
+                                                   frame ? frame->rel : nullptr);
   TSX_CHECK_LAUNCH();
   const double rl2 = r_list * r_list;
   // Fill kernel: per-bin LDS staging pays off for dense lists of many
@@ -1174,17 +1259,20 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
     if (half)
       k_fill<true><<<g_fill, 64 * FILL_W, 0, st>>>(
           sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, w.act, w.bin_start, order, w.wr,
-          w.cw, w.wmax, rl2, self_interaction, jbits, lt, kmax, ld, nbr, count, status);
+          w.cw, w.wmax, rl2, self_interaction, jbits, lt, kmax, ld, nbr, count, status,
+          word_mode);
     else
       k_fill<false><<<g_fill, 64 * FILL_W, 0, st>>>(
           sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, w.act, w.bin_start, order, w.wr,
-          w.cw, w.wmax, rl2, self_interaction, jbits, lt, kmax, ld, nbr, count, status);
+          w.cw, w.wmax, rl2, self_interaction, jbits, lt, kmax, ld, nbr, count, status,
+          word_mode);
   } else {
     // lanes per atom from the row capacity (dense lists: more candidates per
     // column); fewer lanes leave more atoms per wave for the column arithmetic
 #define TSX_FILL_ATOM_LAUNCH(LL, HH)                                                       \
   launch_fill_atom<LL, HH>(st, sys_start, atom_sys, n_atoms, w, bin_cap, order, rl2,       \
-                           self_interaction, jbits, lt, kmax, ld, nbr, count, status, gate)
+                           self_interaction, jbits, lt, kmax, ld, nbr, count, status, gate,   \
+                           word_mode)
     if (kmax >= 64) {
       if (half) TSX_FILL_ATOM_LAUNCH(4, true);
       else TSX_FILL_ATOM_LAUNCH(4, false);
@@ -1252,7 +1340,7 @@ int tsx_wrap_positions(const double* pos, const double* cell_col, const uint8_t*
   ws_layout(0, n_systems, 1, &w, (char*)ws);
   // geometry only: no bin counters are touched (the workspace holds 2)
   k_setup<<<1, 1024, 0, st>>>(cell_col, pbc, pbc_per_system, n_systems, 1.0, 1, 1, 0, w.geom,
-                              w.grid, w.meta, status, w.bin_count, nullptr);
+                              w.grid, w.meta, status, w.bin_count, nullptr, nullptr);
   TSX_CHECK_LAUNCH();
   if (n_atoms == 0) return TSX_OK;
   k_wrap_out<<<blocks_for(n_atoms, 256), 256, 0, st>>>(pos, w.geom, w.grid, atom_sys, n_atoms,

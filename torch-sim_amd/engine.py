@@ -36,6 +36,7 @@ from .models import PairPotentialModel
 from .neighbors import STATUS_BIN_OVERFLOW, STATUS_NL_OVERFLOW, NeighborList, SystemIndex
 
 NVE, LANGEVIN = 0, 1
+WORDS_STENCIL = 1  # TSX_WORDS_STENCIL
 
 
 class BatchedMD:
@@ -121,7 +122,18 @@ class BatchedMD:
         # per-system rebuild gates, one row per step parity
         self.flags = torch.zeros((2, s), dtype=torch.int32, device=dev)
         self.rebuilds = torch.zeros(1, dtype=torch.int32, device=dev)
-        self.nl = NeighborList(model.cutoff + self.skin, tpa=tpa, bins_per_rlist=bins_per_rlist)
+        self.nl = NeighborList(model.cutoff + self.skin, tpa=tpa, bins_per_rlist=bins_per_rlist,
+                               word_mode=WORDS_STENCIL)
+        # bin-relative frame of the stencil-word list: anchors, r - A of each
+        # position buffer (float32, one 16-byte record per neighbour gather),
+        # per-system bin steps
+        self.anchor = torch.zeros((n, 4), dtype=torch.float64, device=dev)
+        self.rel = [torch.zeros((n, 4), dtype=torch.float32, device=dev) for _ in range(2)]
+        self.ustep = torch.zeros((s, 12), dtype=torch.float64, device=dev)
+        self._frames = [
+            _lib.NLFrame(self.anchor.data_ptr(), self.rel[q].data_ptr(), self.ustep.data_ptr())
+            for q in range(2)
+        ]
         self.cur = 0
         self.steps_done = 0
         self._graph = None
@@ -144,7 +156,8 @@ class BatchedMD:
             ptr(self.p), ptr(self.m), ptr(self.f), ptr(self.e_atom), None, self.dt,
             int(self.langevin), ptr(self.kT), ptr(self.gamma), ptr(noise), self.seed,
             ptr(self.step_ctr), int(end_kick), int(do_next), 0.5 * self.skin,
-            ptr(self._gate(1 - q)), stream(),
+            ptr(self._gate(1 - q)), ptr(self.rel[q]), ptr(self.rel[1 - q]), ptr(self.anchor),
+            ptr(self.ustep), stream(),
         )
 
     def _reduce(self, q: int) -> None:
@@ -160,6 +173,7 @@ class BatchedMD:
         self.nl.enqueue(
             self.pos[q], 4, self.cell, self.pbc, self.pbc_per_system, self.sysidx,
             setup=False, gate=self._gate(q), xref=self.xref, counter=self.rebuilds,
+            frame=self._frames[q],
         )
 
     def _step(self, q: int, do_next: bool, noise=None) -> None:
@@ -172,7 +186,8 @@ class BatchedMD:
         """Host-checked list build on the initial positions."""
         if self.n_atoms:
             self.nl.build(
-                self.pos[0], self.cell, self._pbc_bool, self.sysidx, pos_stride=4, xref=self.xref
+                self.pos[0], self.cell, self._pbc_bool, self.sysidx, pos_stride=4, xref=self.xref,
+                frame=self._frames[0],
             )
         self.flags.zero_()
         self.cur = 0
@@ -265,7 +280,7 @@ class BatchedMD:
                 self._graph = None
                 # full rebuild (with geometry) at the restored positions
                 self.nl.build(self.pos[self.cur], self.cell, self._pbc_bool, self.sysidx,
-                              pos_stride=4, xref=self.xref)
+                              pos_stride=4, xref=self.xref, frame=self._frames[self.cur])
                 self.flags.zero_()
                 continue
             if st:
@@ -311,12 +326,13 @@ class BatchedMD:
     def algorithmic_bytes_per_atom_step(self) -> float:
         """Bytes the fused step must move per atom (HBM roofline numerator).
 
-        reads: own packed position 32, momentum 24, mass 8, one 4-byte list
-        word per stored neighbour; writes: position 32, momentum 24, force 24,
-        per-atom energy 8; reduce: per-atom energy 8 (read).
+        reads: own packed position 32, anchor 32, momentum 24, mass 8, one
+        4-byte list word per stored neighbour; writes: position 32, its
+        float32 bin-relative copy 16, momentum 24, force 24, per-atom energy
+        8; reduce: per-atom energy 8 (read).
         """
         k = self.mean_neighbors
-        return 32 + 24 + 8 + 4.0 * k + 32 + 24 + 24 + 8 + 8
+        return 32 + 32 + 24 + 8 + 4.0 * k + 32 + 16 + 24 + 24 + 8 + 8
 
 
 def estimate_skin(cutoff: float) -> float:

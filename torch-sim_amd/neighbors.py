@@ -19,6 +19,7 @@ reused by the models and the batched engine.
 
 from __future__ import annotations
 
+import ctypes
 import math
 
 import numpy as np
@@ -145,9 +146,11 @@ class NeighborList:
         tpa: int = 4,
         bins_per_rlist: int = 1,
         fill_mode: int = 0,
+        word_mode: int = 0,
     ):
         self.r_list = float(r_list)
         self.fill_mode = int(fill_mode)  # TSX_FILL_AUTO / _BIN / _ATOM
+        self.word_mode = int(word_mode)  # TSX_WORDS_SHIFT / _STENCIL
         self.half = bool(half)
         self.self_interaction = bool(self_interaction)
         self.tpa = int(tpa)
@@ -207,11 +210,13 @@ class NeighborList:
         xref: torch.Tensor | None = None,
         counter: torch.Tensor | None = None,
         cell_build: torch.Tensor | None = None,
+        frame: _lib.NLFrame | None = None,
     ) -> None:
         """Enqueue a build on the current stream (capturable, no host sync).
 
         ``gate`` [S] int32 restricts the rebuild to flagged systems; ``setup``
-        recomputes the per-system geometry (needed whenever a cell changed)."""
+        recomputes the per-system geometry (needed whenever a cell changed);
+        ``frame`` receives the bin-relative frame of a stencil-word list."""
         call(
             "tsx_nl_build",
             ptr(pos), pos_stride, ptr(cell_col), ptr(pbc_u8), int(pbc_per_system),
@@ -219,7 +224,8 @@ class NeighborList:
             self.r_list, self.bins_per_rlist, int(self.self_interaction), int(self.half),
             self.jbits, self.tpa, self.kmax, self.ld, self.bin_cap, int(setup), ptr(self.ws),
             ptr(self.nbr), ptr(self.count), ptr(self.order), ptr(xref), ptr(self.status),
-            ptr(gate), ptr(counter), ptr(cell_build), self.fill_mode, stream(),
+            ptr(gate), ptr(counter), ptr(cell_build), self.fill_mode, self.word_mode,
+            None if frame is None else ctypes.addressof(frame), stream(),
         )
 
     def build(
@@ -232,6 +238,7 @@ class NeighborList:
         max_retries: int = 12,
         xref: torch.Tensor | None = None,
         cell_build: torch.Tensor | None = None,
+        frame: _lib.NLFrame | None = None,
     ) -> None:
         """Build with host-side capacity planning and overflow retries."""
         _lib.require_gpu(pos, "positions")
@@ -252,7 +259,7 @@ class NeighborList:
         for _ in range(max_retries):
             self.status.zero_()
             self.enqueue(pos, pos_stride, cell_col, pbc_u8, pbc_per_system, sysidx, xref=xref,
-                         cell_build=cell_build)
+                         cell_build=cell_build, frame=frame)
             st = int(self.status.item())
             if st & STATUS_SINGULAR_CELL:
                 raise _lib.TsxError("periodic system with a singular cell")
