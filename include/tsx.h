@@ -61,6 +61,7 @@ extern "C" {
 #define TSX_STATUS_SHIFT_OVERFLOW 4   /* image shift outside the packed range */
 #define TSX_STATUS_SINGULAR_CELL 8    /* periodic system with singular cell   */
 #define TSX_STATUS_REBUILD 16         /* engine: list must be rebuilt         */
+#define TSX_STATUS_BAD_SIZE 32        /* a system exceeds max_system_atoms    */
 
 /* pair potential kinds */
 #define TSX_LJ 0          /* 4 eps [(s/r)^12 - (s/r)^6]                  */
@@ -110,6 +111,10 @@ const char* tsx_last_error(void);
  * system.  ``xref`` (stride 4, may be NULL) receives a copy of the positions
  * of the rebuilt systems (the displacement reference of the skin) and
  * ``cell_build`` ([S,3,3], may be NULL) a copy of their cells.
+ *
+ * ``max_system_atoms`` (0 = unknown): when every system has at most 2048
+ * atoms, one workgroup per rebuilt system bins, scans, scatters and sorts it
+ * (one launch); otherwise these stages run as grid-wide kernels.
  *
  * ``fill_mode``: TSX_FILL_BIN stages each bin's stencil candidates in LDS
  * (one wave per bin: dense lists, partial rebuilds of many systems);
@@ -161,7 +166,7 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
                  int32_t bin_cap, int32_t setup, void* ws, uint32_t* nbr, int32_t* count,
                  int32_t* order, double* xref, int32_t* status, const int32_t* gate,
                  int32_t* counter, double* cell_build, int32_t fill_mode, int32_t word_mode,
-                 const tsx_nl_frame* frame, void* stream);
+                 const tsx_nl_frame* frame, int32_t max_system_atoms, void* stream);
 
 /* Exclusive scan of ``count`` into ``row_ptr`` [n+1] (int64); the total is
  * row_ptr[n].  Used before tsx_nl_emit_pairs.  ``ws`` holds

@@ -190,6 +190,38 @@ def test_fill_exact_on_edge_cases(dev, mode):
     _assert_same(onl.sort_pairs(*(t.cpu() for t in nl.pairs(sidx, F64))), ref)
 
 
+@pytest.mark.parametrize("case", ["cu", "dilute"])
+def test_one_block_build_matches_grid_build(dev, case):
+    """Systems of <= 2048 atoms are binned and sorted by one workgroup each;
+    the rows equal those of the grid-wide kernels, including a dilute box with
+    more bins than the workgroup counts in LDS (its counters are global)."""
+    if case == "cu":
+        st = tsa.workloads.cu_batch(5, device="cpu", seed=9)
+        rc = 6.0
+        pos, cell_col, pbc, idx, ns = st.positions, st.cell, st.pbc, st.system_idx, st.n_systems
+    else:
+        g = torch.Generator().manual_seed(11)
+        n0, n1 = 150, 90
+        box = torch.tensor([60.0, 55.0, 70.0], dtype=F64)
+        pos = torch.rand(n0 + n1, 3, generator=g, dtype=F64) * box
+        cell_col = torch.diag(box).expand(2, 3, 3).contiguous()
+        pbc = torch.tensor([True, True, True])
+        idx = torch.cat([torch.zeros(n0, dtype=torch.long), torch.ones(n1, dtype=torch.long)])
+        ns, rc = 2, 6.5
+    n = pos.shape[0]
+    si = SystemIndex(idx.cuda(), ns)
+    for q in (1, 2):
+        rows = []
+        for one_block in (True, False):
+            print("build case", case, "q", q, "one_block", one_block, flush=True)
+            nl = NeighborList(rc, tpa=2, bins_per_rlist=q, one_block_build=one_block)
+            nl.build(pos.cuda(), cell_col.cuda().contiguous(), pbc.cuda(), si)
+            rows.append(_raw_rows(nl, n))
+        assert rows[0] == rows[1]
+    ref = onl.brute_force_nl(pos, cell_col.mT, pbc, rc, idx)
+    _assert_same(onl.sort_pairs(*(t.cpu() for t in nl.pairs(si, F64))), ref)
+
+
 def test_wrap_positions_matches_oracle(dev):
     g = GOLD["wrap_batched"]
     cell = torch.tensor(g["cell_row"], dtype=F64)

@@ -132,6 +132,9 @@ struct ExactCtx {
 // is the FP64 one (pair_potential.py:181-194).  The pair function is FP32 on
 // d, accumulation FP64 (pair_potential.py:419-430).  UNR entries at a time:
 // all words, then all gathers in flight.
+// entries per lane with their gathers in flight together
+constexpr int UNR_REL = 8;
+
 template <bool VIRIAL>
 __device__ __forceinline__ void add_pair(RowAcc& a, float e, float fs, float dx, float dy,
                                          float dz) {
@@ -168,16 +171,17 @@ __device__ __forceinline__ RowAcc accumulate_rel(const PairP* sp, int ns, int ti
   const uint32_t* wp = nbr + t * TPA + lane;
   const int64_t stride = ld * TPA;
   const int iters = cnt > lane ? (cnt - lane + TPA - 1) / TPA : 0;
-  for (int k0 = 0; k0 < iters; k0 += UNR) {
-    uint32_t w[UNR];
+  for (int k0 = 0; k0 < iters; k0 += UNR_REL) {
+    uint32_t w[UNR_REL];
 #pragma unroll
-    for (int u = 0; u < UNR; ++u) w[u] = (k0 + u < iters) ? wp[(int64_t)(k0 + u) * stride] : 0u;
-    float4 q[UNR];
+    for (int u = 0; u < UNR_REL; ++u)
+      w[u] = (k0 + u < iters) ? wp[(int64_t)(k0 + u) * stride] : 0u;
+    float4 q[UNR_REL];
 #pragma unroll
-    for (int u = 0; u < UNR; ++u) q[u] = rel_s[w[u] & jmask];
+    for (int u = 0; u < UNR_REL; ++u) q[u] = rel_s[w[u] & jmask];
     uint32_t amb = 0;  // entries within the rounding margin of rc
 #pragma unroll
-    for (int u = 0; u < UNR; ++u) {
+    for (int u = 0; u < UNR_REL; ++u) {
       if (k0 + u >= iters) break;
       const float fx = (float)nl_shift(w[u], jbits, sbits, 0);
       const float fy = (float)nl_shift(w[u], jbits, sbits, 1);
@@ -198,7 +202,9 @@ __device__ __forceinline__ RowAcc accumulate_rel(const PairP* sp, int ns, int ti
     while (amb) {  // rare: the FP64 displacement decides
       const int u = __builtin_ctz(amb);
       amb &= amb - 1u;
-      const uint32_t wu = u == 0 ? w[0] : (u == 1 ? w[1] : (u == 2 ? w[2] : w[UNR - 1]));
+      uint32_t wu = w[0];
+#pragma unroll
+      for (int v = 1; v < UNR_REL; ++v) wu = u == v ? w[v] : wu;
       const uint32_t jl = wu & jmask;
       const double ix = (double)nl_shift(wu, jbits, sbits, 0);
       const double iy = (double)nl_shift(wu, jbits, sbits, 1);

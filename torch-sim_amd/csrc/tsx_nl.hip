@@ -533,6 +533,66 @@ __global__ __launch_bounds__(256) void k_scatter(
 // bin-relative frame (anchor and rel) when asked for, and the bin counter
 // reset for the next build.  Single-bin systems were scattered in atom order
 // already.
+// Everything a build writes for atom i of system s at bin-order place dst:
+// order, wrapped coordinates with the packed wrap shift, the partial list
+// word, and the bin-relative frame (anchor, rel) when asked for.  Returns the
+// atom's largest wrap magnitude.
+__device__ __forceinline__ int emit_atom(int i, int dst, int s, int lb, const WrapOut& o,
+                                         const double* __restrict__ pos, int ps,
+                                         const int32_t* __restrict__ sys_start, const double* g,
+                                         const int32_t* gr, int jbits, int sbits, int stencil,
+                                         int32_t* __restrict__ order, double* __restrict__ wr,
+                                         uint32_t* __restrict__ cw, double* __restrict__ anchor,
+                                         float* __restrict__ rel) {
+  order[dst] = i;
+  double4 v;
+  v.x = o.wr[0];
+  v.y = o.wr[1];
+  v.z = o.wr[2];
+  v.w = pack_w(o.w[0], o.w[1], o.w[2]);
+  reinterpret_cast<double4*>(wr)[dst] = v;
+  const int bz = lb % gr[2], by = (lb / gr[2]) % gr[1], bx = lb / (gr[1] * gr[2]);
+  cw[dst] = (uint32_t)(i - sys_start[s]) |
+            ((stencil ? (uint32_t)bz : pack_fields(-o.w[0], -o.w[1], -o.w[2], sbits)) << jbits);
+  if (anchor || rel) {
+    const double* r = pos + (int64_t)ps * i;
+    double A[3];
+    if (g[22] == 0.0) {  // no lattice: anchor at the system's first atom
+      const double* r0 = pos + (int64_t)ps * sys_start[s];
+      for (int m = 0; m < 3; ++m) A[m] = r0[m];
+    } else {
+      const int bb[3] = {bx, by, bz};
+      double c[3];
+      for (int k = 0; k < 3; ++k) {
+        if (gr[7] & (1 << k)) {
+          c[k] = (double)o.w[k] + (double)bb[k] / (double)gr[k];
+        } else {  // non-periodic direction: the first atom's coordinate
+          const double* r0 = pos + (int64_t)ps * sys_start[s];
+          c[k] = r0[0] * g[9 + k] + r0[1] * g[12 + k] + r0[2] * g[15 + k];
+        }
+      }
+      for (int m = 0; m < 3; ++m) A[m] = c[0] * g[m] + c[1] * g[3 + m] + c[2] * g[6 + m];
+    }
+    if (anchor) {
+      double4 av;
+      av.x = A[0];
+      av.y = A[1];
+      av.z = A[2];
+      av.w = 0.0;
+      reinterpret_cast<double4*>(anchor)[i] = av;
+    }
+    if (rel) {
+      float4 rv;
+      rv.x = (float)(r[0] - A[0]);
+      rv.y = (float)(r[1] - A[1]);
+      rv.z = (float)(r[2] - A[2]);
+      rv.w = ps == 4 ? __int_as_float((int)__double_as_longlong(r[3])) : 0.f;
+      reinterpret_cast<float4*>(rel)[i] = rv;
+    }
+  }
+  return max(abs(o.w[0]), max(abs(o.w[1]), abs(o.w[2])));
+}
+
 __global__ __launch_bounds__(256) void k_sort_atom(
     const double* __restrict__ pos, int ps, const int32_t* __restrict__ sys_start, int n_sys,
     int B, const double* __restrict__ geom, const int32_t* __restrict__ grid,
@@ -561,59 +621,147 @@ __global__ __launch_bounds__(256) void k_sort_atom(
       for (int k = lo; k < hi; ++k) rank += unsorted[k] < i;
       dst = lo + rank;
     }
-    order[dst] = i;
     const WrapOut o = wrap_atom(pos + (int64_t)ps * i, g, gr[7]);
-    double4 v;
-    v.x = o.wr[0];
-    v.y = o.wr[1];
-    v.z = o.wr[2];
-    v.w = pack_w(o.w[0], o.w[1], o.w[2]);
-    reinterpret_cast<double4*>(wr)[dst] = v;
-    const int lb = b - gr[6];
-    const int bz = lb % gr[2], by = (lb / gr[2]) % gr[1], bx = lb / (gr[1] * gr[2]);
-    cw[dst] = (uint32_t)(i - sys_start[s]) |
-              ((stencil ? (uint32_t)bz : pack_fields(-o.w[0], -o.w[1], -o.w[2], sbits)) << jbits);
-    if (anchor || rel) {
-      const double* r = pos + (int64_t)ps * i;
-      double A[3];
-      if (g[22] == 0.0) {  // no lattice: anchor at the system's first atom
-        const double* r0 = pos + (int64_t)ps * sys_start[s];
-        for (int m = 0; m < 3; ++m) A[m] = r0[m];
-      } else {
-        const int bb[3] = {bx, by, bz};
-        double c[3];
-        for (int k = 0; k < 3; ++k) {
-          if (gr[7] & (1 << k)) {
-            c[k] = (double)o.w[k] + (double)bb[k] / (double)gr[k];
-          } else {  // non-periodic direction: the first atom's coordinate
-            const double* r0 = pos + (int64_t)ps * sys_start[s];
-            c[k] = r0[0] * g[9 + k] + r0[1] * g[12 + k] + r0[2] * g[15 + k];
-          }
-        }
-        for (int m = 0; m < 3; ++m) A[m] = c[0] * g[m] + c[1] * g[3 + m] + c[2] * g[6 + m];
-      }
-      if (anchor) {
-        double4 av;
-        av.x = A[0];
-        av.y = A[1];
-        av.z = A[2];
-        av.w = 0.0;
-        reinterpret_cast<double4*>(anchor)[i] = av;
-      }
-      if (rel) {
-        float4 rv;
-        rv.x = (float)(r[0] - A[0]);
-        rv.y = (float)(r[1] - A[1]);
-        rv.z = (float)(r[2] - A[2]);
-        rv.w = ps == 4 ? __int_as_float((int)__double_as_longlong(r[3])) : 0.f;
-        reinterpret_cast<float4*>(rel)[i] = rv;
-      }
-    }
-    wm = max(wm, max(abs(o.w[0]), max(abs(o.w[1]), abs(o.w[2]))));
+    wm = max(wm, emit_atom(i, dst, s, b - gr[6], o, pos, ps, sys_start, g, gr, jbits, sbits,
+                           stencil, order, wr, cw, anchor, rel));
     bin_count[b] = 0;
   }
   for (int off = 32; off > 0; off >>= 1) wm = max(wm, __shfl_xor(wm, off, 64));
   if ((threadIdx.x & 63) == 0 && wm > 0) atomicMax(&wmax[s], wm);
+}
+
+// ------------------------------------------------------ build, one workgroup
+// Bin, scan, scatter and in-bin sort of one system by one workgroup, for
+// batches whose systems have at most SB_MAXN atoms: per-atom bins and slots
+// and the scattered atoms stay in LDS, bin counters too when the system has
+// at most SB_MAXB bins (else the global counters, left cleared).  It writes
+// what bin + scan_sys + scatter + sort_atom write, in the same order, with
+// one launch of one block per system instead of four launches of
+// per-system block groups.
+constexpr int SB_MAXN = 2048;
+constexpr int SB_MAXB = 2048;
+constexpr int SB_THREADS = 512;
+
+// Exclusive scan of v[0..n) in place by the whole block (v[n] = total when
+// write_total).
+__device__ void block_excl_scan(int32_t* v, int n, int32_t* wsum, int32_t* carry,
+                                bool write_total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  constexpr int W = SB_THREADS / 64;
+  constexpr int PER = 4;
+  if (threadIdx.x == 0) *carry = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < n; c0 += SB_THREADS * PER) {
+    const int b0 = c0 + threadIdx.x * PER;
+    int x[PER];
+    int t = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      x[k] = b0 + k < n ? v[b0 + k] : 0;
+      t += x[k];
+    }
+    const int incl = wave_incl_scan(t);
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    int run = *carry + incl - t;
+    for (int q = 0; q < wid; ++q) run += wsum[q];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      if (b0 + k < n) v[b0 + k] = run;
+      run += x[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int tot = 0;
+      for (int q = 0; q < W; ++q) tot += wsum[q];
+      *carry += tot;
+    }
+    __syncthreads();
+  }
+  if (write_total && threadIdx.x == 0) v[n] = *carry;
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(SB_THREADS) void k_build_small(
+    const double* __restrict__ pos, int ps, const int32_t* __restrict__ sys_start, int n_sys,
+    const double* __restrict__ geom, const int32_t* __restrict__ grid,
+    const int32_t* __restrict__ meta, int bin_cap, int32_t* __restrict__ bin_count,
+    int32_t* __restrict__ atom_bin, int32_t* __restrict__ bin_start,
+    int32_t* __restrict__ order, double* __restrict__ wr, const int32_t* __restrict__ gate,
+    double* __restrict__ xref, int32_t* __restrict__ counter,
+    const double* __restrict__ cell_col, double* __restrict__ cell_build, int jbits,
+    uint32_t* __restrict__ cw, int32_t* __restrict__ wmax, int stencil,
+    double* __restrict__ anchor, float* __restrict__ rel, int32_t* __restrict__ status) {
+  __shared__ int32_t lcnt[SB_MAXB + 1];  // bin counts, then exclusive starts
+  __shared__ int32_t abin[SB_MAXN];      // local bin of each atom
+  __shared__ int32_t aslot[SB_MAXN];     // slot of each atom in its bin
+  __shared__ int32_t uns[SB_MAXN];       // atoms scattered to their bins
+  __shared__ int32_t wsum[SB_THREADS / 64];
+  __shared__ int32_t carry;
+  __shared__ int32_t s_wm;
+  const int s = blockIdx.x;
+  if (s >= n_sys || meta[0] > bin_cap || gated_off(gate, s)) return;
+  const int32_t* gr = grid + (size_t)GRID * s;
+  const double* g = geom + (size_t)GEOM * s;
+  const int base = sys_start[s], n_s = sys_start[s + 1] - base;
+  if (n_s > SB_MAXN) {  // max_system_atoms was wrong: refuse, never overrun LDS
+    if (threadIdx.x == 0) atomicOr(status, TSX_STATUS_BAD_SIZE);
+    return;
+  }
+  const int off = gr[6], nb = gr[0] * gr[1] * gr[2];
+  const bool lds = nb <= SB_MAXB;  // else the (cleared) global counters
+  int32_t* cnt = lds ? lcnt : bin_count + off;
+  const int sbits = (32 - jbits) / 3;
+  if (lds)
+    for (int k = threadIdx.x; k <= nb; k += SB_THREADS) lcnt[k] = 0;
+  if (threadIdx.x == 0) s_wm = 0;
+  __syncthreads();
+  for (int a = threadIdx.x; a < n_s; a += SB_THREADS) {
+    const int i = base + a;
+    const double* r = pos + (int64_t)ps * i;
+    const WrapOut o = wrap_atom(r, g, gr[7]);
+    const int b = bin_of(o, gr);
+    abin[a] = b - off;
+    aslot[a] = atomicAdd(&cnt[b - off], 1);
+    atom_bin[i] = b;
+    if (xref) {
+      double4 v;
+      v.x = r[0];
+      v.y = r[1];
+      v.z = r[2];
+      v.w = ps == 4 ? r[3] : 0.0;
+      reinterpret_cast<double4*>(xref)[i] = v;
+    }
+  }
+  __syncthreads();
+  // exclusive bin starts in place; the global counters of this system are
+  // entries [0, nb) only (entry nb is the next system's first bin)
+  block_excl_scan(cnt, nb, wsum, &carry, lds);
+  for (int k = threadIdx.x; k < nb; k += SB_THREADS) bin_start[off + k] = base + cnt[k];
+  if (threadIdx.x == 0) bin_start[off + nb] = base + n_s;
+  for (int a = threadIdx.x; a < n_s; a += SB_THREADS) uns[cnt[abin[a]] + aslot[a]] = a;
+  __syncthreads();
+  int wm = 0;
+  for (int a = threadIdx.x; a < n_s; a += SB_THREADS) {
+    const int lb = abin[a];
+    const int lo = cnt[lb], hi = lb + 1 < nb ? cnt[lb + 1] : n_s;
+    int rank = 0;
+    for (int k = lo; k < hi; ++k) rank += uns[k] < a;
+    const int i = base + a;
+    const WrapOut o = wrap_atom(pos + (int64_t)ps * i, g, gr[7]);
+    wm = max(wm, emit_atom(i, base + lo + rank, s, lb, o, pos, ps, sys_start, g, gr, jbits, sbits,
+                           stencil, order, wr, cw, anchor, rel));
+  }
+  for (int o2 = 32; o2 > 0; o2 >>= 1) wm = max(wm, __shfl_xor(wm, o2, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(&s_wm, wm);
+  __syncthreads();
+  if (!lds)
+    for (int k = threadIdx.x; k < nb; k += SB_THREADS) cnt[k] = 0;
+  if (threadIdx.x == 0) {
+    wmax[s] = s_wm;
+    if (counter) atomicAdd(counter, 1);
+  }
+  if (cell_build && threadIdx.x < 9) cell_build[9 * s + threadIdx.x] = cell_col[9 * s + threadIdx.x];
 }
 
 __device__ __forceinline__ int floordiv(int a, int b) {
@@ -1194,7 +1342,8 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
                  int32_t tpa, int32_t kmax, int64_t ld, int32_t bin_cap, int32_t setup, void* ws,
                  uint32_t* nbr, int32_t* count, int32_t* order, double* xref, int32_t* status,
                  const int32_t* gate, int32_t* counter, double* cell_build, int32_t fill_mode,
-                 int32_t word_mode, const tsx_nl_frame* frame, void* stream) {
+                 int32_t word_mode, const tsx_nl_frame* frame, int32_t max_system_atoms,
+                 void* stream) {
   TSX_REQUIRE(n_systems > 0, "n_systems must be positive");
   TSX_REQUIRE(n_atoms >= 0 && n_atoms < (1ll << 31), "n_atoms out of range");
   TSX_REQUIRE(pos_stride == 3 || pos_stride == 4, "pos_stride must be 3 or 4");
@@ -1222,27 +1371,37 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
     TSX_CHECK_LAUNCH();
   }
   if (n_atoms == 0) return TSX_OK;
-  // blocks per system: whole-grid parallelism for one large system, a few
-  // blocks each for hundreds of small ones
-  const int b_atoms = n_systems >= 2048 ? 1 : 2048 / n_systems;
-  k_bin<<<n_systems * b_atoms, 256, 0, st>>>(pos, pos_stride, sys_start, n_systems, b_atoms,
-                                             w.geom, w.grid, w.meta, bin_cap, w.bin_count,
-                                             w.atom_bin, w.slot, gate, xref);
-  TSX_CHECK_LAUNCH();
-  k_scan_sys<<<n_systems, 256, 0, st>>>(w.grid, w.meta, bin_cap, sys_start, w.bin_count,
-                                        w.bin_start, gate, counter, cell_col, cell_build, w.wmax);
-  TSX_CHECK_LAUNCH();
-  k_scatter<<<n_systems * b_atoms, 256, 0, st>>>(sys_start, n_systems, b_atoms, w.grid, w.meta,
-                                                 bin_cap, w.atom_bin, w.slot, w.bin_start,
-                                                 w.unsorted, gate);
-  TSX_CHECK_LAUNCH();
-  k_sort_atom<<<n_systems * b_atoms, 256, 0, st>>>(pos, pos_stride, sys_start, n_systems, b_atoms,
-                                                   w.geom, w.grid, w.meta, bin_cap, w.atom_bin,
-                                                   w.bin_start, w.unsorted, w.bin_count, order,
-                                                   w.wr, gate, jbits, w.cw, w.wmax, word_mode,
-                                                   frame ? frame->anchor : nullptr,
-                                                   frame ? frame->rel : nullptr);
-  TSX_CHECK_LAUNCH();
+  if (max_system_atoms > 0 && max_system_atoms <= SB_MAXN) {
+    // every system fits one workgroup: bin, scan, scatter and sort in one launch
+    k_build_small<<<n_systems, SB_THREADS, 0, st>>>(
+        pos, pos_stride, sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, w.bin_count,
+        w.atom_bin, w.bin_start, order, w.wr, gate, xref, counter, cell_col, cell_build, jbits,
+        w.cw, w.wmax, word_mode, frame ? frame->anchor : nullptr, frame ? frame->rel : nullptr,
+        status);
+    TSX_CHECK_LAUNCH();
+  } else {
+    // blocks per system: whole-grid parallelism for one large system, a few
+    // blocks each for hundreds of small ones
+    const int b_atoms = n_systems >= 2048 ? 1 : 2048 / n_systems;
+    k_bin<<<n_systems * b_atoms, 256, 0, st>>>(pos, pos_stride, sys_start, n_systems, b_atoms,
+                                               w.geom, w.grid, w.meta, bin_cap, w.bin_count,
+                                               w.atom_bin, w.slot, gate, xref);
+    TSX_CHECK_LAUNCH();
+    k_scan_sys<<<n_systems, 256, 0, st>>>(w.grid, w.meta, bin_cap, sys_start, w.bin_count,
+                                          w.bin_start, gate, counter, cell_col, cell_build, w.wmax);
+    TSX_CHECK_LAUNCH();
+    k_scatter<<<n_systems * b_atoms, 256, 0, st>>>(sys_start, n_systems, b_atoms, w.grid, w.meta,
+                                                   bin_cap, w.atom_bin, w.slot, w.bin_start,
+                                                   w.unsorted, gate);
+    TSX_CHECK_LAUNCH();
+    k_sort_atom<<<n_systems * b_atoms, 256, 0, st>>>(pos, pos_stride, sys_start, n_systems, b_atoms,
+                                                     w.geom, w.grid, w.meta, bin_cap, w.atom_bin,
+                                                     w.bin_start, w.unsorted, w.bin_count, order,
+                                                     w.wr, gate, jbits, w.cw, w.wmax, word_mode,
+                                                     frame ? frame->anchor : nullptr,
+                                                     frame ? frame->rel : nullptr);
+    TSX_CHECK_LAUNCH();
+  }
   const double rl2 = r_list * r_list;
   // Fill kernel: per-bin LDS staging pays off for dense lists of many
   // systems, where gated rebuilds touch a few percent of the atoms and one

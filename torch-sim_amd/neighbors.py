@@ -32,6 +32,7 @@ STATUS_NL_OVERFLOW = 1
 STATUS_BIN_OVERFLOW = 2
 STATUS_SHIFT_OVERFLOW = 4
 STATUS_SINGULAR_CELL = 8
+STATUS_BAD_SIZE = 32
 
 
 class SystemIndex:
@@ -147,10 +148,14 @@ class NeighborList:
         bins_per_rlist: int = 1,
         fill_mode: int = 0,
         word_mode: int = 0,
+        one_block_build: bool = True,
     ):
         self.r_list = float(r_list)
         self.fill_mode = int(fill_mode)  # TSX_FILL_AUTO / _BIN / _ATOM
         self.word_mode = int(word_mode)  # TSX_WORDS_SHIFT / _STENCIL
+        # batches of systems of <= 2048 atoms: bin / scan / sort by one
+        # workgroup per system (False: the grid-wide kernels)
+        self.one_block_build = bool(one_block_build)
         self.half = bool(half)
         self.self_interaction = bool(self_interaction)
         self.tpa = int(tpa)
@@ -225,7 +230,8 @@ class NeighborList:
             self.jbits, self.tpa, self.kmax, self.ld, self.bin_cap, int(setup), ptr(self.ws),
             ptr(self.nbr), ptr(self.count), ptr(self.order), ptr(xref), ptr(self.status),
             ptr(gate), ptr(counter), ptr(cell_build), self.fill_mode, self.word_mode,
-            None if frame is None else ctypes.addressof(frame), stream(),
+            None if frame is None else ctypes.addressof(frame),
+            sysidx.max_atoms if self.one_block_build else 0, stream(),
         )
 
     def build(
@@ -263,6 +269,8 @@ class NeighborList:
             st = int(self.status.item())
             if st & STATUS_SINGULAR_CELL:
                 raise _lib.TsxError("periodic system with a singular cell")
+            if st & STATUS_BAD_SIZE:
+                raise _lib.TsxError("a system has more atoms than max_system_atoms")
             if st & STATUS_SHIFT_OVERFLOW:
                 raise _lib.TsxError(
                     "image shift outside the packed range: positions are too far "
