@@ -866,27 +866,21 @@ __device__ __forceinline__ bool half_keep(const WordCtx& c, uint32_t word, int i
 }
 
 // ------------------------------------------------------------------- fill
-// Persistent grid of four independent waves per block walking the active-bin
-// list (a wave takes one bin at a time).
-// All LDS traffic of a wave stays in its own LDS slice, so waves synchronise
-// with wave barriers only.  For one bin, the stencil images (bin + lattice
-// shift S, m_k bins each way) are enumerated in LDS and their atoms staged in
-// LDS in chunks; L lanes (L * centre-atoms-per-round = 64) share one centre
-// atom: lane ``sub`` tests candidates sub, sub + L, ...; a wave ballot orders
-// the matches so every row lists its neighbours in candidate order -- image
-// order (dx, dy, dz nested), then atom index -- for any L (deterministic).
-constexpr int FILL_W = 4;      // waves per block
-constexpr int FILL_CH = 128;   // staged candidates per chunk
-constexpr int FILL_IMG = 128;  // images per batch
-
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
+// Persistent grid of 256-thread workgroups walking the active-bin list, one
+// bin per workgroup at a time.  For one bin the workgroup enumerates the
+// stencil images (bin + lattice shift S, m_k bins each way, one image per
+// thread), scans their atom counts, and stages the candidates of all images
+// in LDS at once (FILL_CH at a time); a group of L lanes per centre atom
+// (L * centre-atoms-per-round = 256, L <= 64) then tests candidates sub,
+// sub + L, ...; a ballot over the group orders the matches, so every row
+// lists its neighbours in candidate order -- image order (dx, dy, dz
+// nested), then atom index -- for any L (deterministic).
+constexpr int FILL_T = 256;    // threads per block
+constexpr int FILL_CH = 768;   // staged candidates per chunk
+constexpr int FILL_IMG = 128;  // images per batch (one per thread)
 
 template <bool HALF>
-__global__ __launch_bounds__(64 * FILL_W) void k_fill(
+__global__ __launch_bounds__(FILL_T) void k_fill(
     const int32_t* __restrict__ sys_start, int n_sys, const double* __restrict__ geom,
     const int32_t* __restrict__ grid, const int32_t* __restrict__ meta, int bin_cap,
     const int32_t* __restrict__ act, const int32_t* __restrict__ bin_start,
@@ -894,29 +888,22 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
     const uint32_t* __restrict__ cw, const int32_t* __restrict__ wmax, double rl2, int self_int,
     int jbits, int lt, int kmax, int64_t ld, uint32_t* __restrict__ nbr,
     int32_t* __restrict__ count, int32_t* __restrict__ status, int stencil) {
-  __shared__ double4 cpos_all[FILL_W][FILL_CH];
-  __shared__ uint32_t ccw_all[FILL_W][FILL_CH];
-  __shared__ int32_t cimg_all[FILL_W][FILL_CH];
-  __shared__ double ics_all[FILL_W][FILL_IMG][3];
-  __shared__ int32_t is_all[FILL_W][FILL_IMG][3];
-  __shared__ uint32_t sw_all[FILL_W][FILL_IMG];
-  __shared__ int32_t ibeg_all[FILL_W][FILL_IMG];
-  __shared__ int32_t ipre_all[FILL_W][FILL_IMG + 1];
+  __shared__ double4 cpos[FILL_CH];
+  __shared__ uint32_t ccw[FILL_CH];
+  __shared__ int16_t cimg[FILL_CH];
+  __shared__ double ics[FILL_IMG][3];
+  __shared__ int32_t isv[FILL_IMG][3];
+  __shared__ uint32_t sw[FILL_IMG];
+  __shared__ int32_t ibeg[FILL_IMG];
+  __shared__ int32_t ipre[FILL_IMG + 1];
+  __shared__ int32_t wsum[FILL_T / 64];
   if (meta[0] > bin_cap) return;
-  const int wv = threadIdx.x >> 6, tid = threadIdx.x & 63;
-  double4* cpos = cpos_all[wv];
-  uint32_t* ccw = ccw_all[wv];
-  int32_t* cimg = cimg_all[wv];
-  double(*ics)[3] = ics_all[wv];
-  int32_t(*isv)[3] = is_all[wv];
-  uint32_t* sw = sw_all[wv];
-  int32_t* ibeg = ibeg_all[wv];
-  int32_t* ipre = ipre_all[wv];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const WordCtx wc = word_ctx(jbits);
   const int64_t stride = ld << lt;
   const int tmask = (1 << lt) - 1;
   const int total = act[n_sys];
-  for (int wi_ = blockIdx.x * FILL_W + wv; wi_ < total; wi_ += gridDim.x * FILL_W) {
+  for (int wi_ = xcd_block(); wi_ < total; wi_ += gridDim.x) {
     const int s = active_system(act, n_sys, wi_);
     const double* g = geom + (size_t)GEOM * s;
     const int32_t* gr = grid + (size_t)GRID * s;
@@ -927,7 +914,7 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
     const int b = off + (wi_ - act[s]);
     const int c0 = bin_start[b], c1 = bin_start[b + 1];
     const int nc = c1 - c0;
-    if (nc == 0) continue;
+    if (nc == 0) continue;  // uniform over the block
     const bool fast = stencil ? stencil_fits(gr, wc.bias)
                               : stencil_smax(gr) + 2 * wmax[s] < wc.bias;
     if (stencil && !fast) {  // stencil offsets wider than the word fields
@@ -936,12 +923,13 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
     }
     const int local = b - off;
     const int bz0 = local % gr[2], by0 = (local / gr[2]) % gr[1], bx0 = local / (gr[1] * gr[2]);
-    int per = 1;  // centre atoms per round (power of two)
-    while (per < nc && per < 64) per <<= 1;
-    const int L = 64 / per;
+    int per = 4;  // centre atoms per round (power of two, L = FILL_T / per <= 64)
+    while (per < nc && per < FILL_T) per <<= 1;
+    const int L = FILL_T / per;
     const int slot_a = tid / L, sub = tid % L;
-    const uint64_t gmask = (L == 64) ? ~0ull : (((1ull << L) - 1ull) << (slot_a * L));
-    const uint64_t below = (sub == 0) ? 0ull : (((1ull << sub) - 1ull) << (slot_a * L));
+    const int gbase = lane & ~(L - 1);
+    const uint64_t gmask = (L == 64) ? ~0ull : (((1ull << L) - 1ull) << gbase);
+    const uint64_t below = ((1ull << sub) - 1ull) << gbase;
     for (int r0 = c0; r0 < c1; r0 += per) {  // rounds of centre atoms
       const int t = r0 + slot_a;
       const bool own = t < c1;
@@ -959,12 +947,10 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
       bool shift_ovf = false;
       for (int ib0 = 0; ib0 < n_img; ib0 += FILL_IMG) {
         const int nib = min(FILL_IMG, n_img - ib0);
-        wave_sync();
-        int lsum = 0;
-        for (int h = 0; h < 2; ++h) {  // images 2 * tid, 2 * tid + 1
-          const int q = 2 * tid + h;
-          if (q >= nib) continue;
-          const int im = ib0 + q;
+        __syncthreads();  // previous users of the image and candidate tables are done
+        int n_q = 0;
+        if (tid < nib) {  // image tid of this batch
+          const int im = ib0 + tid;
           const int dz = im % wz - gr[5];
           const int dy = (im / wz) % wy - gr[4];
           const int dx = im / (wy * wz) - gr[3];
@@ -973,35 +959,32 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
           const int nbin = off + ((cx - sx * gr[0]) * gr[1] + (cy - sy * gr[1])) * gr[2] +
                            (cz - sz * gr[2]);
           // S . L (transforms.py:563-565)
-          ics[q][0] = shift_comp(sx, sy, sz, g, 0);
-          ics[q][1] = shift_comp(sx, sy, sz, g, 1);
-          ics[q][2] = shift_comp(sx, sy, sz, g, 2);
-          isv[q][0] = sx;
-          isv[q][1] = sy;
-          isv[q][2] = sz;
-          sw[q] = (stencil ? pack_fields(dx, dy, dz, wc.sbits) : pack_fields(sx, sy, sz, wc.sbits))
-                  << jbits;
+          ics[tid][0] = shift_comp(sx, sy, sz, g, 0);
+          ics[tid][1] = shift_comp(sx, sy, sz, g, 1);
+          ics[tid][2] = shift_comp(sx, sy, sz, g, 2);
+          isv[tid][0] = sx;
+          isv[tid][1] = sy;
+          isv[tid][2] = sz;
+          sw[tid] = (stencil ? pack_fields(dx, dy, dz, wc.sbits) : pack_fields(sx, sy, sz, wc.sbits))
+                    << jbits;
           const int kb = bin_start[nbin], ke = bin_start[nbin + 1];
-          ibeg[q] = kb;
-          ipre[q + 1] = ke - kb;
-          lsum += ke - kb;
+          ibeg[tid] = kb;
+          n_q = ke - kb;
         }
-        // exclusive prefix of the image sizes: lane-pair sums, wave scan
-        const int incl = wave_incl_scan(lsum);
-        const int excl = incl - lsum;
-        wave_sync();
-        if (2 * tid < nib) {
-          const int n0 = ipre[2 * tid + 1];
-          ipre[2 * tid] = excl;
-          if (2 * tid + 1 < nib) ipre[2 * tid + 1] = excl + n0;
-        }
-        if (tid == 63) ipre[nib] = incl;
-        wave_sync();
+        // exclusive prefix of the image sizes over the block
+        const int incl = wave_incl_scan(n_q);
+        if (lane == 63) wsum[wid] = incl;
+        __syncthreads();
+        int excl = incl - n_q;
+        for (int q = 0; q < wid; ++q) excl += wsum[q];
+        if (tid < nib) ipre[tid] = excl;
+        if (tid == FILL_T - 1) ipre[nib] = excl + n_q;
+        __syncthreads();
         const int ncand = ipre[nib];
         for (int k0 = 0; k0 < ncand; k0 += FILL_CH) {
           const int nk = min(FILL_CH, ncand - k0);
-          wave_sync();
-          for (int c = tid; c < nk; c += 64) {
+          if (k0 > 0) __syncthreads();  // the previous chunk's tests are done
+          for (int c = tid; c < nk; c += FILL_T) {
             const int gc = k0 + c;
             int lo = 0, hi = nib - 1;  // image holding candidate gc
             while (lo < hi) {
@@ -1014,10 +997,10 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
             const int k = ibeg[lo] + (gc - ipre[lo]);
             cpos[c] = reinterpret_cast<const double4*>(wr)[k];
             ccw[c] = cw[k];
-            cimg[c] = lo;
+            cimg[c] = (int16_t)lo;
           }
-          wave_sync();
-          for (int cb = 0; cb < nk; cb += L) {  // uniform trip count over the wave
+          __syncthreads();
+          for (int cb = 0; cb < nk; cb += L) {  // uniform trip count over the block
             const int c = cb + sub;
             bool match = false;
             uint32_t word = 0;
@@ -1063,6 +1046,7 @@ __global__ __launch_bounds__(64 * FILL_W) void k_fill(
         count[t] = cnt;
       }
     }
+    __syncthreads();  // the next bin rewrites the tables
   }
 }
 
@@ -1414,14 +1398,14 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
   if (per_bin) {
     k_active<<<1, 1024, 0, st>>>(w.grid, w.meta, bin_cap, n_systems, gate, w.act);
     TSX_CHECK_LAUNCH();
-    const int g_fill = blocks_for(bin_cap, FILL_W) < 768 ? blocks_for(bin_cap, FILL_W) : 768;
+    const int g_fill = bin_cap < 1024 ? bin_cap : 1024;
     if (half)
-      k_fill<true><<<g_fill, 64 * FILL_W, 0, st>>>(
+      k_fill<true><<<g_fill, FILL_T, 0, st>>>(
           sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, w.act, w.bin_start, order, w.wr,
           w.cw, w.wmax, rl2, self_interaction, jbits, lt, kmax, ld, nbr, count, status,
           word_mode);
     else
-      k_fill<false><<<g_fill, 64 * FILL_W, 0, st>>>(
+      k_fill<false><<<g_fill, FILL_T, 0, st>>>(
           sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, w.act, w.bin_start, order, w.wr,
           w.cw, w.wmax, rl2, self_interaction, jbits, lt, kmax, ld, nbr, count, status,
           word_mode);
