@@ -225,3 +225,36 @@ def test_skin_list_gradients_equal_rebuilt_list(dev):
         out.append((xn.detach(), sig.grad.clone(), eps.grad.clone()))
     for a, b in zip(*out):
         torch.testing.assert_close(a, b, rtol=1e-11, atol=1e-11)
+
+
+@pytest.mark.parametrize("skin", [0.0, 0.3])
+def test_nve_graph_segments_equal_eager(dev, skin):
+    """Segments replayed from captured HIP graphs (captured on the first
+    trajectory of a geometry, replayed on the next ones, eager work between
+    them) give bitwise the trajectory and gradients of the same kernels
+    launched one by one on a fresh geometry; uneven last segment included
+    (30 steps, checkpoints every 7)."""
+    st = tsa.workloads.binary_lj_batch(2, n_min=32, n_max=48, device=dev, seed=12)
+    model = tsa.MultiLennardJonesModel(torch.tensor([1, 2]), KA_SIG, KA_EPS, device=dev, cutoff=RC)
+    g = torch.Generator().manual_seed(5)
+    p0 = (torch.randn(st.n_atoms, 3, generator=g, dtype=F64) * 0.7).to(dev)
+    c = torch.randn(st.n_atoms, 3, generator=g, dtype=F64).to(dev)
+
+    def run(geo, use_graph):
+        sig = KA_SIG.to(dev).clone().requires_grad_(True)
+        eps = KA_EPS.to(dev).clone().requires_grad_(True)
+        params = tad.pair_params(sig, eps, torch.zeros(2, 2, dtype=F64, device=dev))
+        xn, pn = tad.nve_trajectory(st.positions, p0, params, geo, st.masses, st.cell, dt=0.002,
+                                    n_steps=30, checkpoint_every=7, use_graph=use_graph)
+        loss = tad.pair_energy(xn, st.cell, params, geo).sum() + (c * pn).sum()
+        loss.backward()
+        torch.cuda.synchronize()
+        return xn.detach().clone(), pn.detach().clone(), sig.grad.clone(), eps.grad.clone()
+
+    ref = run(tad.geometry(tad.PairTerms(model, skin=skin), st), False)
+    geo = tad.geometry(tad.PairTerms(model, skin=skin), st)
+    for rep in range(3):
+        print("graph replay", rep, "skin", skin, flush=True)
+        out = run(geo, True)
+        for a, b in zip(ref, out):
+            assert torch.equal(a, b), rep

@@ -68,11 +68,16 @@ class Geometry:
         self.n_atoms = int(positions.shape[0])
         self.rebuild(positions, cell)
 
-    def rebuild(self, positions, cell) -> None:
+    def rebuild(self, positions, cell, *, full: bool = False) -> None:
+        """Make the list valid at ``positions`` (``full``: rebuild every
+        system, so the list no longer depends on earlier calls)."""
         pos = positions.detach().to(torch.float64).contiguous()
         cl = cell.detach().to(torch.float64).contiguous()
         if self.terms.vl is not None:
-            self.terms.vl.update(pos, cl, self.pbc, self.sysidx)
+            if full:
+                self.terms.vl.rebuild_all(pos, cl, self.pbc, self.sysidx)
+            else:
+                self.terms.vl.update(pos, cl, self.pbc, self.sysidx)
         else:
             self.terms.nl.build(pos, cl, self.pbc, self.sysidx)
 
@@ -207,77 +212,226 @@ def pair_energy(positions, cell, params, geo: Geometry) -> torch.Tensor:
     return _PairEnergy.apply(positions, cell, params.to(positions.device), geo)
 
 
+class _TrajRunner:
+    """Static buffers and (optionally) captured HIP graphs for an NVE
+    trajectory and its adjoint on one :class:`Geometry`.
+
+    A segment of velocity-Verlet steps (forward), the recomputation of a
+    segment from its checkpoint (storing every position), and the reverse
+    sweep of a segment (force VJP + adjoint update per step) each run without
+    a host synchronisation: the list is kept valid by the capturable skin
+    check and gated rebuild, and its capacity status is read once per
+    segment.  On an overflow the runner grows the list, rebuilds it at the
+    segment's start and redoes the segment eagerly.  Every kernel sequence is
+    the same with and without graphs, so results are bitwise equal."""
+
+    def __init__(self, geo: Geometry, masses, cell, dt: float, use_graph: bool):
+        self.geo, self.dt, self.use_graph = geo, float(dt), bool(use_graph)
+        n, s, ns = geo.n_atoms, geo.n_systems, geo.terms.n_species
+        dev = masses.device
+        z3 = lambda: torch.zeros((n, 3), dtype=torch.float64, device=dev)  # noqa: E731
+        self.x, self.p, self.f = z3(), z3(), z3()
+        self.xbar, self.pbar, self.w, self.vx = z3(), z3(), z3(), z3()
+        self.e = torch.zeros(max(1, n), dtype=torch.float64, device=dev)
+        self.m = _f64(masses).clone()
+        self.cl = _f64(cell).clone()
+        self.par = torch.zeros((ns, ns, 3), dtype=torch.float64, device=dev)
+        self.dpar = torch.zeros_like(self.par)
+        self.pbc_u8 = geo.pbc.to(torch.uint8).contiguous()
+        self.pbc_per = geo.pbc.ndim == 2
+        nl = geo.terms.nl
+        self.gws = torch.zeros(max(1, int(_lib.lib().tsx_grad_ws_bytes(n, nl.tpa, ns)) // 8 + 1),
+                               dtype=torch.float64, device=dev)
+        self.xs = None
+        self.graphs: dict = {}
+        self.gen = None
+
+    # kernel sequences (no allocation, no host read) -------------------------
+    def _update(self, pos):
+        geo, t = self.geo, self.geo.terms
+        # the cell is constant along a trajectory: the per-system grids of the
+        # host-checked build at its start stay valid (setup=False)
+        if t.vl is not None:
+            t.vl.enqueue_update(pos, self.cl, self.pbc_u8, self.pbc_per, geo.sysidx, setup=False)
+        else:
+            t.nl.enqueue(pos, 3, self.cl, self.pbc_u8, self.pbc_per, geo.sysidx, setup=False)
+
+    def _forces(self, pos):
+        geo, nl = self.geo, self.geo.terms.nl
+        call("tsx_pair_forces", geo.terms.kind, ptr(self.par), geo.terms.n_species,
+             ptr(geo.species), ptr(pos), 3, ptr(self.cl), ptr(geo.sysidx.sys_start),
+             ptr(geo.sysidx.atom_sys), ptr(nl.nbr), ptr(nl.count), ptr(nl.order), geo.n_atoms,
+             nl.jbits, nl.tpa, nl.ld, geo.terms.cutoff, 0, ptr(self.f), ptr(self.e), None,
+             stream())
+
+    def _vjp(self, pos, want_par: bool):
+        geo, nl = self.geo, self.geo.terms.nl
+        call("tsx_pair_grad", geo.terms.kind, ptr(self.par), geo.terms.n_species,
+             ptr(geo.species), ptr(pos), 3, ptr(self.cl), ptr(geo.sysidx.sys_start),
+             ptr(geo.sysidx.atom_sys), geo.n_systems, ptr(nl.nbr), ptr(nl.count), ptr(nl.order),
+             geo.n_atoms, nl.jbits, nl.tpa, nl.ld, geo.terms.cutoff, ptr(self.w), None,
+             ptr(self.vx), None, ptr(self.dpar) if want_par else None, 1, ptr(self.gws),
+             stream())
+
+    def _vv(self):
+        n, dt = self.geo.n_atoms, self.dt
+        call("tsx_md_vv_half", ptr(self.x), ptr(self.p), ptr(self.f), ptr(self.m), dt, n, stream())
+        self._update(self.x)
+        self._forces(self.x)
+        call("tsx_md_kick", ptr(self.p), ptr(self.f), 0.5 * dt, n, stream())
+
+    def _seq_forward(self, k: int):
+        for _ in range(k):
+            self._vv()
+
+    def _seq_recompute(self, k: int):
+        self._update(self.x)
+        self._forces(self.x)
+        for j in range(k):
+            n, dt = self.geo.n_atoms, self.dt
+            call("tsx_md_vv_half", ptr(self.x), ptr(self.p), ptr(self.f), ptr(self.m), dt, n,
+                 stream())
+            self.xs[j].copy_(self.x)
+            self._update(self.x)
+            self._forces(self.x)
+            call("tsx_md_kick", ptr(self.p), ptr(self.f), 0.5 * dt, n, stream())
+
+    def _seq_reverse(self, k: int, first: bool, want_par: bool):
+        n, dt = self.geo.n_atoms, self.dt
+        for j in range(k, 0, -1):
+            xn = self.xs[j - 1]
+            self._update(xn)
+            self._vjp(xn, want_par)
+            # the global step 1 is last: the next VJP is at x_0 (half kick only)
+            wscale = 0.5 * dt if (first and j == 1) else dt
+            call("tsx_nve_adjoint_update", ptr(self.xbar), ptr(self.pbar), ptr(self.vx),
+                 ptr(self.m), dt, wscale, ptr(self.w), n, stream())
+
+    # execution ---------------------------------------------------------------
+    def _generation(self):
+        t = self.geo.terms
+        return t.vl.generation if t.vl is not None else t.nl.generation
+
+    def _status(self) -> int:
+        return int(self.geo.terms.nl.status.item())
+
+    def _run(self, key, fn, *args, graph_ok: bool = True):
+        gen = self._generation()
+        if gen != self.gen:  # a buffer a graph points to was replaced
+            self.graphs.clear()
+            self.gen = gen
+        self.geo.terms.nl.status.zero_()
+        if not (self.use_graph and graph_ok):
+            fn(*args)
+            return
+        g = self.graphs.get(key)
+        if g is None:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                fn(*args)
+            self.graphs[key] = g
+        g.replay()
+
+    def _recover(self, st: int, pos):
+        t = self.geo.terms
+        if t.vl is not None:
+            t.vl.recover(st, pos, self.cl, self.geo.pbc, self.geo.sysidx)
+        else:
+            t.nl.build(pos, self.cl, self.geo.pbc, self.geo.sysidx)
+
+    def segment(self, kind: str, k: int, *, first: bool = False, want_par: bool = False):
+        """Run one segment; redo it eagerly after growing the list when the
+        status word reports an overflow."""
+        snap = (self.x.clone(), self.p.clone(), self.f.clone(), self.xbar.clone(),
+                self.pbar.clone(), self.w.clone(), self.dpar.clone())
+        for attempt in range(8):
+            if kind == "fwd":
+                self._run(("fwd", k), self._seq_forward, k, graph_ok=attempt == 0)
+            elif kind == "rec":
+                self._run(("rec", k), self._seq_recompute, k, graph_ok=attempt == 0)
+            else:
+                self._run(("rev", k, first, want_par), self._seq_reverse, k, first, want_par,
+                          graph_ok=attempt == 0)
+            st = self._status()
+            if not st:
+                return
+            for dst, src in zip((self.x, self.p, self.f, self.xbar, self.pbar, self.w, self.dpar),
+                                snap):
+                dst.copy_(src)
+            start = self.xs[k - 1] if kind == "rev" else self.x
+            self._recover(st, start)
+        raise _lib.TsxError("neighbor list capacity could not be resolved in a trajectory")
+
+
 class _NVETrajectory(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x0, p0, params, geo: Geometry, masses, cell, dt: float, n_steps: int,
-                checkpoint_every: int):
-        x, p, par = _f64(x0).clone(), _f64(p0).clone(), _f64(params)
-        m, cl = _f64(masses), _f64(cell)
-        n = x.shape[0]
+                checkpoint_every: int, use_graph: bool):
+        # one runner (static buffers, captured segments) per geometry and
+        # (dt, use_graph); its inputs are reloaded at every forward / backward
+        key = (float(dt), bool(use_graph))
+        cache = geo.__dict__.setdefault("_traj_runners", {})
+        r = cache.get(key)
+        if r is None:
+            r = cache[key] = _TrajRunner(geo, masses, cell, dt, use_graph)
+        par, m, cl = _f64(params).clone(), _f64(masses).clone(), _f64(cell).clone()
+        r.par.copy_(par)
+        r.m.copy_(m)
+        r.cl.copy_(cl)
+        r.x.copy_(_f64(x0))
+        r.p.copy_(_f64(p0))
+        geo.rebuild(r.x, r.cl, full=True)  # every system at x_0: history-independent
+        r._forces(r.x)
         ckpts = []
-        geo.rebuild(x, cl)
-        f = geo.forces(x, cl, par)
-        for k in range(n_steps):
-            if k % checkpoint_every == 0:
-                ckpts.append((k, x.clone(), p.clone()))
-            call("tsx_md_vv_half", ptr(x), ptr(p), ptr(f), ptr(m), dt, n, stream())
-            geo.rebuild(x, cl)
-            f = geo.forces(x, cl, par)
-            call("tsx_md_kick", ptr(p), ptr(f), 0.5 * dt, n, stream())
-        ctx.geo, ctx.dt, ctx.n_steps = geo, dt, n_steps
-        ctx.ckpts = ckpts
-        ctx.save_for_backward(par, m, cl)
-        return x, p
+        for k0 in range(0, n_steps, checkpoint_every):
+            ckpts.append((k0, r.x.clone(), r.p.clone()))
+            r.segment("fwd", min(checkpoint_every, n_steps - k0))
+        ctx.runner, ctx.n_steps, ctx.ckpts = r, n_steps, ckpts
+        ctx.inputs = (par, m, cl)
+        return r.x.clone(), r.p.clone()
 
     @staticmethod
     def backward(ctx, xbar_n, pbar_n):
-        par, m, cl = ctx.saved_tensors
-        geo, dt, n_steps = ctx.geo, ctx.dt, ctx.n_steps
-        n = m.shape[0]
-        dev = m.device
-        xbar = _f64(xbar_n).clone() if xbar_n is not None else torch.zeros((n, 3), dtype=torch.float64, device=dev)
-        pbar = _f64(pbar_n).clone() if pbar_n is not None else torch.zeros((n, 3), dtype=torch.float64, device=dev)
-        dpar = torch.zeros_like(par)
+        r, n_steps, ckpts = ctx.runner, ctx.n_steps, ctx.ckpts
         want_par = ctx.needs_input_grad[2]
-        w = 0.5 * dt * pbar  # cotangent of F(x_N): only the last half kick
-        for seg in reversed(range(len(ctx.ckpts))):
-            k0, xs, ps = ctx.ckpts[seg]
-            k1 = ctx.ckpts[seg + 1][0] if seg + 1 < len(ctx.ckpts) else n_steps
-            # recompute x_{k0+1 .. k1} from the checkpoint
-            xs_list = []
-            x, p = xs.clone(), ps.clone()
-            geo.rebuild(x, cl)
-            f = geo.forces(x, cl, par)
-            for _ in range(k0, k1):
-                call("tsx_md_vv_half", ptr(x), ptr(p), ptr(f), ptr(m), dt, n, stream())
-                xs_list.append(x.clone())
-                geo.rebuild(x, cl)
-                f = geo.forces(x, cl, par)
-                call("tsx_md_kick", ptr(p), ptr(f), 0.5 * dt, n, stream())
-            for step in range(k1, k0, -1):
-                xn = xs_list[step - k0 - 1]
-                geo.rebuild(xn, cl)
-                vx, _, _ = geo.grad(xn, cl, par, w=w, want_pos=True, want_params=want_par,
-                                    dparams_acc=dpar if want_par else None)
-                # step - 1 == 0: the next VJP is at x_0 (first half kick only)
-                call("tsx_nve_adjoint_update", ptr(xbar), ptr(pbar), ptr(vx), ptr(m), dt,
-                     dt if step - 1 > 0 else 0.5 * dt, ptr(w), n, stream())
-        x0 = ctx.ckpts[0][1]
-        geo.rebuild(x0, cl)
-        vx, _, _ = geo.grad(x0, cl, par, w=w, want_pos=True, want_params=want_par,
-                            dparams_acc=dpar if want_par else None)
-        xbar = xbar + vx
-        return xbar, pbar, (dpar if want_par else None), None, None, None, None, None, None
+        for dst, src in zip((r.par, r.m, r.cl), ctx.inputs):
+            dst.copy_(src)
+        seg_max = max(min(ckpts[i + 1][0] if i + 1 < len(ckpts) else n_steps, n_steps) - ckpts[i][0]
+                      for i in range(len(ckpts)))
+        if r.xs is None or r.xs.shape[0] < seg_max:
+            r.xs = torch.zeros((seg_max, r.geo.n_atoms, 3), dtype=torch.float64,
+                               device=r.x.device)
+            r.graphs.clear()  # the recompute and reverse graphs point to xs
+        r.xbar.copy_(_f64(xbar_n)) if xbar_n is not None else r.xbar.zero_()
+        r.pbar.copy_(_f64(pbar_n)) if pbar_n is not None else r.pbar.zero_()
+        r.dpar.zero_()
+        torch.mul(r.pbar, 0.5 * r.dt, out=r.w)  # cotangent of F(x_N): the last half kick only
+        for seg in reversed(range(len(ckpts))):
+            k0, xs, ps = ckpts[seg]
+            k1 = ckpts[seg + 1][0] if seg + 1 < len(ckpts) else n_steps
+            r.x.copy_(xs)
+            r.p.copy_(ps)
+            r.segment("rec", k1 - k0)
+            r.segment("rev", k1 - k0, first=k0 == 0, want_par=want_par)
+        # the VJP at x_0 (first half kick)
+        r.x.copy_(ckpts[0][1])
+        r.geo.rebuild(r.x, r.cl)
+        r._vjp(r.x, want_par)
+        xbar = r.xbar + r.vx
+        return (xbar, r.pbar.clone(), (r.dpar.clone() if want_par else None), None, None, None,
+                None, None, None, None)
 
 
 def nve_trajectory(x0, p0, params, geo: Geometry, masses, cell, *, dt: float, n_steps: int,
-                   checkpoint_every: int = 20):
+                   checkpoint_every: int = 20, use_graph: bool = True):
     """(x_n, p_n) after ``n_steps`` velocity-Verlet steps, differentiable in
-    x0, p0 and params through the checkpointed discrete adjoint."""
+    x0, p0 and params through the checkpointed discrete adjoint.  Each
+    segment of ``checkpoint_every`` steps runs as one captured HIP graph
+    (``use_graph=False``: the same kernels launched one by one)."""
     if n_steps < 1 or checkpoint_every < 1:
         raise ValueError("n_steps and checkpoint_every must be >= 1")
     return _NVETrajectory.apply(x0, p0, params.to(x0.device), geo, masses, cell, float(dt),
-                                int(n_steps), int(checkpoint_every))
+                                int(n_steps), int(checkpoint_every), bool(use_graph))
 
 
 __all__ = [

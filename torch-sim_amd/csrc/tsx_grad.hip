@@ -258,20 +258,45 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_pair_grad(
   }
 }
 
-// Sum the block partials in order and symmetrise: out[a,b] = out[b,a] =
+// Sum the block partials and symmetrise: out[a,b] = out[b,a] =
 // (P[a,b] + P[b,a]) / 2, so that the sum of the two entries is the derivative
-// with respect to the shared parameter of the unordered species pair.
-__global__ void k_theta_reduce(const double* __restrict__ part, int n_blocks, int ns,
-                               double* __restrict__ out, int accumulate) {
-  const int k = threadIdx.x;
+// with respect to the shared parameter of the unordered species pair.  One
+// workgroup: thread t adds blocks t, t + 256, ... for every entry, then a
+// fixed-order tree over the workgroup (deterministic).
+constexpr int THETA_T = 256;
+constexpr int THETA_MAX = GRAD_MAX_NS * GRAD_MAX_NS * 3;
+
+__global__ __launch_bounds__(THETA_T) void k_theta_reduce(const double* __restrict__ part,
+                                                          int n_blocks, int ns,
+                                                          double* __restrict__ out,
+                                                          int accumulate) {
   const int nb = ns * ns * 3;
-  __shared__ double tot[GRAD_MAX_NS * GRAD_MAX_NS * 3];
-  if (k < nb) {
-    double v = 0.0;
-    for (int b = 0; b < n_blocks; ++b) v += part[(int64_t)b * nb + k];
-    tot[k] = v;
+  __shared__ double red[THETA_T / 64][THETA_MAX];
+  __shared__ double tot[THETA_MAX];
+  double acc[THETA_MAX];
+#pragma unroll
+  for (int k = 0; k < THETA_MAX; ++k) acc[k] = 0.0;
+  for (int b = threadIdx.x; b < n_blocks; b += THETA_T) {
+    const double* pb = part + (int64_t)b * nb;
+#pragma unroll
+    for (int k = 0; k < THETA_MAX; ++k)
+      if (k < nb) acc[k] += pb[k];
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < THETA_MAX; ++k) {
+    if (k >= nb) break;
+    const double t = wave_sum(acc[k]);
+    if (lane == 0) red[wid][k] = t;
   }
   __syncthreads();
+  if (threadIdx.x < nb) {
+    double v = 0.0;
+    for (int q = 0; q < THETA_T / 64; ++q) v += red[q][threadIdx.x];
+    tot[threadIdx.x] = v;
+  }
+  __syncthreads();
+  const int k = threadIdx.x;
   if (k < nb) {
     const int a = k / (3 * ns), b = (k / 3) % ns, p = k % 3;
     const double v = 0.5 * (tot[k] + tot[(b * ns + a) * 3 + p]);
@@ -436,7 +461,7 @@ int tsx_pair_grad(int32_t kind, const double* params, int32_t n_species, const i
   TSX_REQUIRE(rc == 0, "unsupported launch configuration");
   TSX_CHECK_LAUNCH();
   if (dparams) {
-    k_theta_reduce<<<1, 64, 0, st>>>(dth, nblk, ns, dparams, accumulate);
+    k_theta_reduce<<<1, THETA_T, 0, st>>>(dth, nblk, ns, dparams, accumulate);
     TSX_CHECK_LAUNCH();
   }
   if (dcell_out) {

@@ -166,6 +166,7 @@ class NeighborList:
         self.n_atoms = -1
         self.n_systems = -1
         self.nbr = self.count = self.order = self.ws = self.status = None
+        self.generation = 0  # bumped whenever the buffers are reallocated
 
     # ------------------------------------------------------------ sizing
     def ensure(self, n_atoms: int, n_systems: int, kmax: int, bin_cap: int, jbits: int,
@@ -183,6 +184,7 @@ class NeighborList:
         self.jbits = jbits
         if not realloc:
             return
+        self.generation += 1
         self.kmax = max(kmax, self.kmax if n_atoms == self.n_atoms else 0)
         self.bin_cap = max(bin_cap, self.bin_cap if n_systems == self.n_systems else 0)
         self.n_atoms, self.n_systems = n_atoms, n_systems
@@ -335,6 +337,7 @@ class VerletList:
         self.xref = self.cell_build = self.flags = self.ws = None
         self.full_builds = 0
         self.updates = 0
+        self._gen = 0
 
     def _full(self, pos, cell_col, pbc, sysidx, pos_stride):
         dev = pos.device
@@ -347,6 +350,55 @@ class VerletList:
                       cell_build=self.cell_build)
         self._key = (sysidx._key, sysidx.n_atoms, sysidx.n_systems)
         self.full_builds += 1
+        self._gen += 1
+
+    @property
+    def generation(self) -> tuple:
+        """Changes whenever a buffer a captured graph points to is replaced."""
+        return (self._gen, self.nl.generation)
+
+    def ready(self, sysidx: SystemIndex) -> bool:
+        return self.xref is not None and self._key == (sysidx._key, sysidx.n_atoms,
+                                                        sysidx.n_systems)
+
+    def rebuild_all(self, pos: torch.Tensor, cell_col: torch.Tensor, pbc: torch.Tensor,
+                    sysidx: SystemIndex, pos_stride: int = 3) -> NeighborList:
+        """Host-checked build of every system at ``pos`` (the list state then
+        depends on ``pos`` only), keeping the buffers when they fit."""
+        if not self.ready(sysidx):
+            self._full(pos, cell_col, pbc, sysidx, pos_stride)
+        else:
+            self.nl.build(pos, cell_col, pbc, sysidx, pos_stride, xref=self.xref,
+                          cell_build=self.cell_build)
+        return self.nl
+
+    def enqueue_update(self, pos: torch.Tensor, cell_col: torch.Tensor, pbc_u8: torch.Tensor,
+                       pbc_per_system: bool, sysidx: SystemIndex, pos_stride: int = 3,
+                       setup: bool = True) -> None:
+        """The skin check and gated rebuild of :meth:`update` without the host
+        read (capturable; the list must be ready; ``setup=False`` keeps the
+        per-system grids of the last setup build, for cells that do not change).  Capacity problems
+        accumulate in ``nl.status``: the caller checks it after a run of
+        updates and, on overflow, calls :meth:`recover` and redoes the run."""
+        if sysidx.n_atoms == 0:
+            return
+        call("tsx_nl_check", ptr(pos), pos_stride, ptr(cell_col), ptr(self.cell_build),
+             ptr(self.xref), ptr(sysidx.atom_sys), sysidx.n_atoms, sysidx.n_systems,
+             self.nl.r_list, self.cutoff, ptr(self.flags), ptr(self.ws), stream())
+        self.nl.enqueue(pos, pos_stride, cell_col, pbc_u8, pbc_per_system, sysidx, setup=setup,
+                        gate=self.flags, xref=self.xref, cell_build=self.cell_build)
+
+    def recover(self, status: int, pos, cell_col, pbc, sysidx, pos_stride: int = 3) -> None:
+        """After an overflow seen in ``status``: more room, then a full build
+        at ``pos`` (raises for a singular cell or an oversized system)."""
+        if status & STATUS_SINGULAR_CELL:
+            raise _lib.TsxError("periodic system with a singular cell")
+        if status & STATUS_BAD_SIZE:
+            raise _lib.TsxError("a system has more atoms than max_system_atoms")
+        kmax = self.nl.kmax * 2 if status & STATUS_NL_OVERFLOW else self.nl.kmax
+        bins = self.nl.bin_cap * 4 if status & STATUS_BIN_OVERFLOW else self.nl.bin_cap
+        self.nl.ensure(sysidx.n_atoms, sysidx.n_systems, kmax, bins, self.nl.jbits, pos.device)
+        self._full(pos, cell_col, pbc, sysidx, pos_stride)
 
     def update(self, pos: torch.Tensor, cell_col: torch.Tensor, pbc: torch.Tensor,
                sysidx: SystemIndex, pos_stride: int = 3) -> NeighborList:
