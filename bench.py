@@ -40,10 +40,10 @@ MORSE = dict(sigma=2.866, epsilon=0.3429, alpha=1.3588)
 # WRITE_SIZE x 1.000).  Keyed by (workload, skin, tpa); the summaries with their commands
 # are profiles/k_md_fused__cu_morse_nvt_512sys.md and profiles/k_md_fused__lj_fluid_1m_nve.md.
 MEASURED_TRAFFIC = {
-    ("cu_morse_nvt", 0.72, 2): {"fetch_kib": 107098.9, "fetch_factor": 2.000,
-                                "write_kib": 35924.8, "write_factor": 1.000},
-    ("lj1m_nve", 0.3, 4): {"fetch_kib": 307970.8, "fetch_factor": 2.000,
-                           "write_kib": 112049.8, "write_factor": 1.000},
+    ("cu_morse_nvt", 0.72, 2): {"fetch_kib": 122803.6, "fetch_factor": 2.000,
+                                "write_kib": 41604.3, "write_factor": 1.000},
+    ("lj1m_nve", 0.3, 4): {"fetch_kib": 343661.2, "fetch_factor": 2.000,
+                           "write_kib": 125743.0, "write_factor": 1.000},
 }
 
 
@@ -209,15 +209,16 @@ def plain_torch(state, steps, warmup):
     return state.n_atoms * steps / (time.perf_counter() - t0)
 
 
-def cpu_baseline(n_sys=12, steps=2, seed=0):
+def cpu_baseline(n_sys=32, min_seconds=10.0, max_steps=200, seed=0):
     """FP64 CPU oracle (reference algorithm: linked-cell list rebuilt every
     step, autograd half-list forces, BAOAB) on the first ``n_sys`` systems of
-    the same workload."""
+    the same workload, stepped until ``min_seconds`` of CPU work have run (a
+    bounded sample of the workload)."""
     from oracle import md as omd
     from oracle import models as om
     from oracle import nl as onl
 
-    cores = os.cpu_count() or 1
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     torch.set_num_threads(cores)
     st = tsa.workloads.cu_batch(512, device="cpu", seed=seed)
     cnt = torch.bincount(st.system_idx)
@@ -242,9 +243,11 @@ def cpu_baseline(n_sys=12, steps=2, seed=0):
     e, f = force(pos)  # warm-up / init
     x = pos
     t0 = time.perf_counter()
-    for _ in range(steps):
+    steps = 0
+    while steps < max_steps and (steps < 2 or time.perf_counter() - t0 < min_seconds):
         noise = torch.randn(n_atoms, 3, generator=g, dtype=torch.float64)
         x, p, f, e = omd.langevin_step(x, p, f, m, idx, force, dt, kT, gamma, noise)
+        steps += 1
     el = time.perf_counter() - t0
     return {
         "value": n_atoms * steps / el,
@@ -573,7 +576,7 @@ def main() -> None:
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-systems", type=int, default=12)
+    ap.add_argument("--cpu-systems", type=int, default=32)
     ap.add_argument("--fit-md-steps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
