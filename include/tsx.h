@@ -117,7 +117,8 @@ const char* tsx_last_error(void);
  * (one launch); otherwise these stages run as grid-wide kernels.
  *
  * ``fill_mode``: TSX_FILL_BIN stages each bin's stencil candidates in LDS
- * (one wave per bin: dense lists, partial rebuilds of many systems);
+ * (one 256-thread workgroup per bin: dense lists, partial rebuilds of many
+ * systems);
  * TSX_FILL_ATOM gives each atom a group of lanes reading its stencil columns
  * from L2 (sparse bins, full builds of large systems); TSX_FILL_AUTO picks
  * BIN for bins_per_rlist >= 2, kmax >= 64 and more than one system.  Both

@@ -326,13 +326,17 @@ class BatchedMD:
     def algorithmic_bytes_per_atom_step(self) -> float:
         """Bytes the fused step must move per atom (HBM roofline numerator).
 
-        reads: own packed position 32, anchor 32, momentum 24, mass 8, one
-        4-byte list word per stored neighbour; writes: position 32, its
-        float32 bin-relative copy 16, momentum 24, force 24, per-atom energy
-        8; reduce: per-atom energy 8 (read).
+        reads: own packed position 32, anchor 32, displacement reference 32,
+        the float32 bin-relative position 16 (every atom's record is gathered
+        at least once), momentum 24, mass 8, row order 4, system index 4, row
+        length 4, one 4-byte list word per stored neighbour; writes: position
+        32, its float32 bin-relative copy 16, momentum 24, force 24, per-atom
+        energy 8; reduce: per-atom energy 8 (read).
         """
         k = self.mean_neighbors
-        return 32 + 32 + 24 + 8 + 4.0 * k + 32 + 16 + 24 + 24 + 8 + 8
+        reads = 32 + 32 + 32 + 16 + 24 + 8 + 4 + 4 + 4 + 4.0 * k
+        writes = 32 + 16 + 24 + 24 + 8
+        return reads + writes + 8
 
 
 def estimate_skin(cutoff: float) -> float:
