@@ -876,8 +876,23 @@ __device__ __forceinline__ bool half_keep(const WordCtx& c, uint32_t word, int i
 // lists its neighbours in candidate order -- image order (dx, dy, dz
 // nested), then atom index -- for any L (deterministic).
 constexpr int FILL_T = 256;    // threads per block
-constexpr int FILL_CH = 768;   // staged candidates per chunk
+// 512 staged candidates (a dense bin stages about 500) keep one workgroup's
+// LDS near 25 KB, so six workgroups share a CU: the fill is bound by the
+// latency of its per-bin phases, and resident workgroups hide it.
+constexpr int FILL_CH = 512;   // staged candidates per chunk
 constexpr int FILL_IMG = 128;  // images per batch (one per thread)
+
+// Inclusive prefix sum over the wave of non-negative v, one ballot per bit
+// of the largest v: exact, and no lane-address registers.
+__device__ __forceinline__ int wave_incl_scan_bits(int v) {
+  const uint64_t le = (2ull << (threadIdx.x & 63)) - 1ull;  // lanes <= this one
+  int r = 0;
+  for (int b = 0; b < 31; ++b) {
+    if (__ballot((v >> b) != 0) == 0) break;
+    r += __popcll(__ballot((v >> b) & 1) & le) << b;
+  }
+  return r;
+}
 
 template <bool HALF>
 __global__ __launch_bounds__(FILL_T) void k_fill(
@@ -972,7 +987,7 @@ __global__ __launch_bounds__(FILL_T) void k_fill(
           n_q = ke - kb;
         }
         // exclusive prefix of the image sizes over the block
-        const int incl = wave_incl_scan(n_q);
+        const int incl = wave_incl_scan_bits(n_q);
         if (lane == 63) wsum[wid] = incl;
         __syncthreads();
         int excl = incl - n_q;
@@ -1184,6 +1199,23 @@ __global__ __launch_bounds__(256) void k_fill_atom(
   count[t] = cnt;
 }
 
+// Workgroups of the per-bin fill that fit on the device at once: its grid is
+// persistent, so a larger grid only adds a second, serial round of
+// workgroups.  Queried once per device and HALF variant.
+template <bool HALF>
+static int fill_resident_blocks() {
+  static int cache[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 1024;
+  if (cache[dev] > 0) return cache[dev];
+  int cus = 0, per = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_fill<HALF>, FILL_T, 0) != hipSuccess ||
+      cus <= 0 || per <= 0)
+    return 1024;
+  cache[dev] = cus * per;
+  return cache[dev];
+}
 template <int L, bool HALF>
 static void launch_fill_atom(hipStream_t st, const int32_t* sys_start, const int32_t* atom_sys,
                              int64_t n, const NLWs& w, int bin_cap, const int32_t* order,
@@ -1398,7 +1430,8 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
   if (per_bin) {
     k_active<<<1, 1024, 0, st>>>(w.grid, w.meta, bin_cap, n_systems, gate, w.act);
     TSX_CHECK_LAUNCH();
-    const int g_fill = bin_cap < 1024 ? bin_cap : 1024;
+    const int g_res = half ? fill_resident_blocks<true>() : fill_resident_blocks<false>();
+    const int g_fill = bin_cap < g_res ? bin_cap : g_res;
     if (half)
       k_fill<true><<<g_fill, FILL_T, 0, st>>>(
           sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, w.act, w.bin_start, order, w.wr,
