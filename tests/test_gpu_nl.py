@@ -190,13 +190,15 @@ def test_fill_exact_on_edge_cases(dev, mode):
     _assert_same(onl.sort_pairs(*(t.cpu() for t in nl.pairs(sidx, F64))), ref)
 
 
-@pytest.mark.parametrize("case", ["cu", "dilute"])
+@pytest.mark.parametrize("case", ["cu", "dilute", "many"])
 def test_one_block_build_matches_grid_build(dev, case):
     """Systems of <= 2048 atoms are binned and sorted by one workgroup each;
     the rows equal those of the grid-wide kernels, including a dilute box with
-    more bins than the workgroup counts in LDS (its counters are global)."""
-    if case == "cu":
-        st = tsa.workloads.cu_batch(5, device="cpu", seed=9)
+    more bins than the workgroup counts in LDS (its counters are global).
+    Batches of at most 16 systems take the chunked bin scan on the grid path,
+    larger ones ("many", 20 systems) the one-workgroup-per-system scan."""
+    if case in ("cu", "many"):
+        st = tsa.workloads.cu_batch(5 if case == "cu" else 20, device="cpu", seed=9)
         rc = 6.0
         pos, cell_col, pbc, idx, ns = st.positions, st.cell, st.pbc, st.system_idx, st.n_systems
     else:

@@ -504,6 +504,97 @@ __global__ __launch_bounds__(256) void k_scan_sys(
   if (cell_build && threadIdx.x < 9) cell_build[9 * s + threadIdx.x] = cell_col[9 * s + threadIdx.x];
 }
 
+// The same per-system scan in three launches for batches of a few large
+// systems (one workgroup would walk every bin of a 10^5-bin system): chunk
+// scans, then a scan of the chunk totals per system, then the offsets.
+// Bit-identical to k_scan_sys (an integer prefix).
+constexpr int SCAN_CHUNKS = 256;  // chunks per system
+
+__global__ __launch_bounds__(256) void k_scan_chunk(
+    const int32_t* __restrict__ grid, const int32_t* __restrict__ meta, int bin_cap,
+    const int32_t* __restrict__ bin_count, int32_t* __restrict__ bin_start,
+    const int32_t* __restrict__ gate, int32_t* __restrict__ part) {
+  __shared__ int32_t wsum[4];
+  __shared__ int32_t carry;
+  const int s = blockIdx.x / SCAN_CHUNKS, j = blockIdx.x % SCAN_CHUNKS;
+  if (meta[0] > bin_cap || gated_off(gate, s)) return;
+  const int32_t* gr = grid + (size_t)GRID * s;
+  const int off = gr[6], nb = gr[0] * gr[1] * gr[2];
+  const int chunk = (nb + SCAN_CHUNKS - 1) / SCAN_CHUNKS;
+  const int lo = min(nb, j * chunk), hi = min(nb, lo + chunk);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  constexpr int TILE = 256 * SCAN_PER;
+  for (int c0 = lo; c0 < hi; c0 += TILE) {
+    const int base = c0 + threadIdx.x * SCAN_PER;
+    int v[SCAN_PER];
+    int tsum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+      v[k] = base + k < hi ? bin_count[off + base + k] : 0;
+      tsum += v[k];
+    }
+    const int incl = wave_incl_scan(tsum);
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    int run = carry + incl - tsum;
+    for (int q = 0; q < wid; ++q) run += wsum[q];
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+      if (base + k < hi) bin_start[off + base + k] = run;
+      run += v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = carry;
+}
+
+__global__ __launch_bounds__(SCAN_CHUNKS) void k_scan_parts(
+    const int32_t* __restrict__ grid, const int32_t* __restrict__ meta, int bin_cap,
+    const int32_t* __restrict__ sys_start, int32_t* __restrict__ bin_start,
+    const int32_t* __restrict__ gate, int32_t* __restrict__ counter,
+    const double* __restrict__ cell_col, double* __restrict__ cell_build,
+    int32_t* __restrict__ wmax, int32_t* __restrict__ part) {
+  __shared__ int32_t wsum[SCAN_CHUNKS / 64];
+  const int s = blockIdx.x;
+  if (meta[0] > bin_cap || gated_off(gate, s)) return;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int v = part[s * SCAN_CHUNKS + threadIdx.x];
+  const int incl = wave_incl_scan(v);
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  int run = sys_start[s] + incl - v;
+  for (int q = 0; q < wid; ++q) run += wsum[q];
+  part[s * SCAN_CHUNKS + threadIdx.x] = run;
+  if (threadIdx.x == 0) {
+    // Systems' bins are contiguous, so the end of system s is the first bin
+    // of system s + 1, which k_scan_add completes from its local prefix; a
+    // write here would be added to.  Only the last system's end is free.
+    const int32_t* gr = grid + (size_t)GRID * s;
+    if (s == (int)gridDim.x - 1) bin_start[gr[6] + gr[0] * gr[1] * gr[2]] = sys_start[s + 1];
+    wmax[s] = 0;  // raised by the sort of this build
+    if (counter) atomicAdd(counter, 1);
+  }
+  if (cell_build && threadIdx.x < 9) cell_build[9 * s + threadIdx.x] = cell_col[9 * s + threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(
+    const int32_t* __restrict__ grid, const int32_t* __restrict__ meta, int bin_cap,
+    int32_t* __restrict__ bin_start, const int32_t* __restrict__ gate,
+    const int32_t* __restrict__ part) {
+  const int s = blockIdx.x / SCAN_CHUNKS, j = blockIdx.x % SCAN_CHUNKS;
+  if (meta[0] > bin_cap || gated_off(gate, s)) return;
+  const int32_t* gr = grid + (size_t)GRID * s;
+  const int off = gr[6], nb = gr[0] * gr[1] * gr[2];
+  const int chunk = (nb + SCAN_CHUNKS - 1) / SCAN_CHUNKS;
+  const int lo = min(nb, j * chunk), hi = min(nb, lo + chunk);
+  const int add = part[blockIdx.x];
+  for (int b = lo + (int)threadIdx.x; b < hi; b += 256) bin_start[off + b] += add;
+}
+
 __global__ __launch_bounds__(256) void k_scatter(
     const int32_t* __restrict__ sys_start, int n_sys, int B, const int32_t* __restrict__ grid,
     const int32_t* __restrict__ meta, int bin_cap, const int32_t* __restrict__ atom_bin,
@@ -1403,9 +1494,23 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
                                                w.geom, w.grid, w.meta, bin_cap, w.bin_count,
                                                w.atom_bin, w.slot, gate, xref);
     TSX_CHECK_LAUNCH();
-    k_scan_sys<<<n_systems, 256, 0, st>>>(w.grid, w.meta, bin_cap, sys_start, w.bin_count,
-                                          w.bin_start, gate, counter, cell_col, cell_build, w.wmax);
-    TSX_CHECK_LAUNCH();
+    if (n_systems <= 4096 / SCAN_CHUNKS) {  // a few large systems: chunked scan
+      k_scan_chunk<<<n_systems * SCAN_CHUNKS, 256, 0, st>>>(w.grid, w.meta, bin_cap, w.bin_count,
+                                                             w.bin_start, gate, w.scan_tmp);
+      TSX_CHECK_LAUNCH();
+      k_scan_parts<<<n_systems, SCAN_CHUNKS, 0, st>>>(w.grid, w.meta, bin_cap, sys_start,
+                                                      w.bin_start, gate, counter, cell_col,
+                                                      cell_build, w.wmax, w.scan_tmp);
+      TSX_CHECK_LAUNCH();
+      k_scan_add<<<n_systems * SCAN_CHUNKS, 256, 0, st>>>(w.grid, w.meta, bin_cap, w.bin_start,
+                                                           gate, w.scan_tmp);
+      TSX_CHECK_LAUNCH();
+    } else {
+      k_scan_sys<<<n_systems, 256, 0, st>>>(w.grid, w.meta, bin_cap, sys_start, w.bin_count,
+                                            w.bin_start, gate, counter, cell_col, cell_build,
+                                            w.wmax);
+      TSX_CHECK_LAUNCH();
+    }
     k_scatter<<<n_systems * b_atoms, 256, 0, st>>>(sys_start, n_systems, b_atoms, w.grid, w.meta,
                                                    bin_cap, w.atom_bin, w.slot, w.bin_start,
                                                    w.unsorted, gate);
