@@ -731,7 +731,7 @@ __global__ __launch_bounds__(256) void k_sort_atom(
 // per-system block groups.
 constexpr int SB_MAXN = 2048;
 constexpr int SB_MAXB = 2048;
-constexpr int SB_THREADS = 512;
+constexpr int SB_THREADS = 1024;
 
 // Exclusive scan of v[0..n) in place by the whole block (v[n] = total when
 // write_total).
