@@ -234,8 +234,15 @@ __device__ __forceinline__ RowAcc accumulate_rel(const PairP* sp, int ns, int ti
   return a;
 }
 
+// LJ and Morse without the virial: four waves per SIMD.  Their unconstrained
+// allocation lands at 130-132 VGPRs (three waves), and the gather-bound walk
+// gains more from the fourth wave's loads in flight than it loses to the
+// tighter budget (they fit it without spills).  The virial and soft-sphere
+// variants keep the default budget: capped, they would spill.
 template <int KIND, int TPA, bool LANGEVIN, bool VIRIAL, bool MULTI>
-__global__ __launch_bounds__(256) void k_md_fused(FusedArgs A) {
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(VIRIAL || KIND == TSX_SOFT_SPHERE ? 1 : 4)))
+void k_md_fused(FusedArgs A) {
   __shared__ PairP sp[MAX_SPECIES_MD * MAX_SPECIES_MD];
   for (int k = threadIdx.x; k < A.ns * A.ns; k += blockDim.x)
     sp[k] = make_pair_params(KIND, A.params[3 * k], A.params[3 * k + 1], A.params[3 * k + 2]);
