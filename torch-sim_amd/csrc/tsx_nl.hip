@@ -1165,9 +1165,12 @@ __global__ __launch_bounds__(FILL_T) void k_fill(
 // are read from L2: rows of one bin share their candidates, so the reads
 // coalesce.  No LDS and no staging, so sparse bins cost only their
 // candidates, and partial (gated) rebuilds still fill the chip with L lanes
-// per rebuilt atom.
+// per rebuilt atom.  Four lanes per atom (dense rows, full builds of large
+// systems) run five waves per SIMD (96 VGPRs, 2-3 spilled); two lanes keep
+// the default four.
 template <int L, bool HALF>
-__global__ __launch_bounds__(256) void k_fill_atom(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 4 ? 5 : 1)))
+void k_fill_atom(
     const int32_t* __restrict__ sys_start, const int32_t* __restrict__ atom_sys,
     int64_t n_atoms, const double* __restrict__ geom, const int32_t* __restrict__ grid,
     const int32_t* __restrict__ meta, int bin_cap, const int32_t* __restrict__ atom_bin,
