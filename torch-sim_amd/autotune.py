@@ -83,22 +83,33 @@ def tune_md(
     st = state.clone()
     st.rng = 0
     md = integrators.nvt_langevin_init(state=st, model=model, kT=kT)
-    best = None
+    def rate_of(skin, tpa, steps):
+        eng = BatchedMD(model, md, dt=dt, kT=kT, langevin=True, skin=skin, tpa=tpa, seed=1,
+                        chunk_steps=steps)
+        eng.run(warmup_steps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.run(steps)
+        torch.cuda.synchronize()
+        return state.n_atoms * steps / (time.perf_counter() - t0)
+
+    trials = []
     for skin in skins:
         for tpa in tpas:
-            eng = BatchedMD(model, md, dt=dt, kT=kT, langevin=True, skin=skin, tpa=tpa, seed=1,
-                            chunk_steps=trial_steps)
-            eng.run(warmup_steps)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            eng.run(trial_steps)
-            torch.cuda.synchronize()
-            rate = state.n_atoms * trial_steps / (time.perf_counter() - t0)
+            rate = rate_of(skin, tpa, trial_steps)
             if log:
                 log(f"  autotune skin={skin} tpa={tpa}: {rate:.3e} atom-steps/s")
-            if best is None or rate > best[0]:
-                best = (rate, skin, tpa)
-            del eng
+            trials.append((rate, skin, tpa))
+    # the two fastest again, over twice the steps: neighbouring skins are
+    # often within the noise of one short trial
+    trials.sort(reverse=True)
+    best = None
+    for _, skin, tpa in trials[:2]:
+        rate = rate_of(skin, tpa, 2 * trial_steps)
+        if log:
+            log(f"  autotune re-timed skin={skin} tpa={tpa}: {rate:.3e} atom-steps/s")
+        if best is None or rate > best[0]:
+            best = (rate, skin, tpa)
     cfg = {"skin": best[1], "tpa": best[2], "rate": best[0]}
     _MEM[key] = cfg
     _store(key, cfg)

@@ -226,7 +226,8 @@ class PairPotentialModel(ModelInterface):
         si = self._system_index(state)
         pbc = state.pbc
         if self._vl is not None:
-            nl = self._vl.update(pos, cell, pbc, si)
+            # the build status is read once, after the force kernels are queued
+            nl = self._vl.update(pos, cell, pbc, si, defer_status=True)
         else:
             nl = self._nl
             nl.build(pos, cell, pbc, si)
@@ -236,7 +237,8 @@ class PairPotentialModel(ModelInterface):
         need_w = self._compute_stress
         w_atom = torch.empty((n, 6), dtype=torch.float64, device=dev) if need_w else None
         species = self.species_of(state.atomic_numbers)
-        if n:
+
+        def pair_forces(nl):
             call(
                 "tsx_pair_forces",
                 self.kind, ptr(self._params), self.n_species, ptr(species), ptr(pos), 3,
@@ -244,6 +246,11 @@ class PairPotentialModel(ModelInterface):
                 ptr(nl.order), n, nl.jbits, nl.tpa, nl.ld, self.cutoff, 0, ptr(forces),
                 ptr(e_atom), ptr(w_atom), stream(),
             )
+
+        if n:
+            pair_forces(nl)
+            if self._vl is not None and self._vl.settle():
+                pair_forces(self._vl.nl)  # the list was rebuilt with more room
         energy = torch.empty(s, dtype=torch.float64, device=dev)
         stress = torch.empty((s, 3, 3), dtype=torch.float64, device=dev) if need_w else None
         stresses = (

@@ -401,9 +401,19 @@ class VerletList:
         self._full(pos, cell_col, pbc, sysidx, pos_stride)
 
     def update(self, pos: torch.Tensor, cell_col: torch.Tensor, pbc: torch.Tensor,
-               sysidx: SystemIndex, pos_stride: int = 3) -> NeighborList:
-        """Make the list valid for ``pos`` / ``cell_col``; returns it."""
+               sysidx: SystemIndex, pos_stride: int = 3,
+               defer_status: bool = False) -> NeighborList:
+        """Make the list valid for ``pos`` / ``cell_col``; returns it.
+
+        With ``defer_status`` the skin check and the gated rebuild are only
+        queued: the caller queues its work on the returned list, then calls
+        :meth:`settle`, which reads the build status once (one host
+        synchronisation per call instead of one between the build and the
+        force kernels) and says whether that work must be redone on a list
+        rebuilt with more room.
+        """
         self.updates += 1
+        self._pending = None
         key = (sysidx._key, sysidx.n_atoms, sysidx.n_systems)
         if self._key != key or self.xref is None:
             self._full(pos, cell_col, pbc, sysidx, pos_stride)
@@ -417,6 +427,20 @@ class VerletList:
         self.nl.enqueue(pos, pos_stride, cell_col, pbc.to(torch.uint8).contiguous(), pbc.ndim == 2,
                         sysidx, setup=True, gate=self.flags, xref=self.xref,
                         cell_build=self.cell_build)
+        if defer_status:
+            self._pending = (pos, cell_col, pbc, sysidx, pos_stride)
+            return self.nl
+        self._settle(pos, cell_col, pbc, sysidx, pos_stride)
+        return self.nl
+
+    def settle(self) -> bool:
+        """Read the status of a deferred :meth:`update`.  Returns True when
+        the list had to be rebuilt (every system, with more room), so that
+        results computed from it since the update must be computed again."""
+        pending, self._pending = getattr(self, "_pending", None), None
+        return False if pending is None else self._settle(*pending)
+
+    def _settle(self, pos, cell_col, pbc, sysidx, pos_stride) -> bool:
         st = int(self.nl.status.item())
         if st & STATUS_SINGULAR_CELL:
             raise _lib.TsxError("periodic system with a singular cell")
@@ -429,7 +453,8 @@ class VerletList:
                 self.nl.ensure(sysidx.n_atoms, sysidx.n_systems, self.nl.kmax,
                                self.nl.bin_cap * 4, self.nl.jbits, pos.device)
             self._full(pos, cell_col, pbc, sysidx, pos_stride)
-        return self.nl
+            return True
+        return False
 
 
 def _normalize(cell: torch.Tensor, pbc: torch.Tensor, n_systems: int):

@@ -261,8 +261,13 @@ def fire_step(
     cf = state.cell_filter if cellf else None
     first = False
     if fire_flavor == "ase_fire":
-        # fire.py:313-315: the raw-force first step when every velocity is NaN
-        first = bool(state.velocities.isnan().any(dim=1).all())
+        # fire.py:313-315: the raw-force first step when every velocity is NaN.
+        # Velocities that this function's previous call left, untouched since
+        # (same tensor, same in-place version), are not all NaN: skip the
+        # reduction and its host synchronisation.
+        vel, seen = state.velocities, getattr(state, "_fire_velocities_seen", None)
+        if seen is None or seen[0] is not vel or seen[1] != vel._version:
+            first = bool(vel.isnan().any(dim=1).all())
     prm = _params(fire_flavor, cf, dt_max=dt_max, n_min=n_min, f_inc=f_inc, f_dec=f_dec,
                   alpha_start=alpha_start, f_alpha=f_alpha, max_step=max_step,
                   hydrostatic=state.hydrostatic_strain if cellf else False,
@@ -292,6 +297,7 @@ def fire_step(
     if "stress" in out:
         state.stress = out["stress"]
     _post(state, prm, init=False)
+    state._fire_velocities_seen = (state.velocities, state.velocities._version)
     return state
 
 
