@@ -278,7 +278,11 @@ int tsx_md_baoa(double* x, double* p, const double* f, const double* m, const in
  * rel_in = the rel of pos_in): each neighbour costs one 16-byte gather of
  * rel; the FP32 displacement decides the cutoff outside a proven rounding
  * margin and the FP64 displacement (r_j - A_j) - (r_i - A_i) + d.ustep
- * decides inside it.  rel_out receives the rel of pos_out. */
+ * decides inside it.  rel_out receives the rel of pos_out.
+ * ``orthorhombic`` nonzero promises that every system's cell is orthorhombic
+ * (the off-diagonal entries of cell_col, hence of ustep, are exactly zero):
+ * the stencil shift d.ustep then costs one product per component, with the
+ * same bits as the general form.  Pass 0 when unsure. */
 int tsx_md_fused_step(int32_t kind, const double* params, int32_t n_species,
                       const double* pos_in, double* pos_out, const double* xref,
                       const double* cell_col, const int32_t* sys_start,
@@ -289,7 +293,8 @@ int tsx_md_fused_step(int32_t kind, const double* params, int32_t n_species,
                       const double* kT, const double* gamma, const double* noise,
                       uint64_t seed, const int64_t* step, int32_t end_kick, int32_t do_next,
                       double half_skin, int32_t* flag_set, const float* rel_in, float* rel_out,
-                      const double* anchor, const double* ustep, void* stream);
+                      const double* anchor, const double* ustep, int32_t orthorhombic,
+                      void* stream);
 
 /* Per-system energy [S] = sum of e_atom (the chunked fixed-order reduction
  * of tsx_system_reduce, same workspace); gate_clear[s] = 0 when given (the

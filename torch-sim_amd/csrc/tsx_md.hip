@@ -112,6 +112,7 @@ struct FusedArgs {
   float* rel_out;       // [N,4] r - A of pos_out
   const double* anchor; // [N,4] bin anchors A
   const double* ustep;  // [S,12] a_k / nb_k (rows), sum_k |a_k / nb_k|
+  int ortho;            // every cell orthorhombic (ustep off-diagonals zero)
 };
 
 // The FP64 side of the cutoff decision for entries within the rounding
@@ -155,7 +156,26 @@ __device__ __forceinline__ void add_pair(RowAcc& a, float e, float fs, float dx,
   }
 }
 
-template <int KIND, int TPA, bool VIRIAL, bool MULTI>
+// d . ustep for a stencil offset d, per component.  With ORTHO (every cell
+// orthorhombic: ustep rows have zero off-diagonal entries) one product per
+// component, rounded as the general FMA chain rounds it when those entries
+// are zero (an FMA with a zero factor returns its addend): bitwise the same.
+template <bool ORTHO>
+__device__ __forceinline__ void stencil_disp(float fx, float fy, float fz, const float* uf,
+                                             float& sx, float& sy, float& sz) {
+#pragma clang fp contract(off)
+  if (ORTHO) {
+    sx = fx * uf[0];
+    sy = fy * uf[4];
+    sz = fz * uf[8];
+  } else {
+    sx = fmaf(fz, uf[6], fmaf(fy, uf[3], fx * uf[0]));
+    sy = fmaf(fz, uf[7], fmaf(fy, uf[4], fx * uf[1]));
+    sz = fmaf(fz, uf[8], fmaf(fy, uf[5], fx * uf[2]));
+  }
+}
+
+template <int KIND, int TPA, bool VIRIAL, bool MULTI, bool ORTHO>
 __device__ __forceinline__ RowAcc accumulate_rel(const PairP* sp, int ns, int ti,
                                                  const float4* __restrict__ rel_s, float xri,
                                                  float yri, float zri, const float* uf,
@@ -186,9 +206,11 @@ __device__ __forceinline__ RowAcc accumulate_rel(const PairP* sp, int ns, int ti
       const float fx = (float)nl_shift(w[u], jbits, sbits, 0);
       const float fy = (float)nl_shift(w[u], jbits, sbits, 1);
       const float fz = (float)nl_shift(w[u], jbits, sbits, 2);
-      const float dx = (q[u].x - xri) + fmaf(fz, uf[6], fmaf(fy, uf[3], fx * uf[0]));
-      const float dy = (q[u].y - yri) + fmaf(fz, uf[7], fmaf(fy, uf[4], fx * uf[1]));
-      const float dz = (q[u].z - zri) + fmaf(fz, uf[8], fmaf(fy, uf[5], fx * uf[2]));
+      float sx, sy, sz;
+      stencil_disp<ORTHO>(fx, fy, fz, uf, sx, sy, sz);
+      const float dx = (q[u].x - xri) + sx;
+      const float dy = (q[u].y - yri) + sy;
+      const float dz = (q[u].z - zri) + sz;
       const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
       if (!(d2 < hi2)) continue;
       if (!(d2 < lo2)) {
@@ -239,7 +261,7 @@ __device__ __forceinline__ RowAcc accumulate_rel(const PairP* sp, int ns, int ti
 // gains more from the fourth wave's loads in flight than it loses to the
 // tighter budget (they fit it without spills).  The virial and soft-sphere
 // variants keep the default budget: capped, they would spill.
-template <int KIND, int TPA, bool LANGEVIN, bool VIRIAL, bool MULTI>
+template <int KIND, int TPA, bool LANGEVIN, bool VIRIAL, bool MULTI, bool ORTHO>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(VIRIAL || KIND == TSX_SOFT_SPHERE ? 1 : 4)))
 void k_md_fused(FusedArgs A) {
@@ -282,7 +304,7 @@ void k_md_fused(FusedArgs A) {
   ex.rc2 = A.rc * A.rc;
   const float4* rel_s = reinterpret_cast<const float4*>(A.rel_in) + base;
   const int cnt = valid ? A.count[tt] : 0;
-  const RowAcc a = accumulate_rel<KIND, TPA, VIRIAL, MULTI>(
+  const RowAcc a = accumulate_rel<KIND, TPA, VIRIAL, MULTI, ORTHO>(
       sp, A.ns, ti, rel_s, (float)ex.xi, (float)ex.yi, (float)ex.zi, uf, A.nbr, tt, cnt, lane,
       A.jbits, A.ld, lo2, hi2, ex);
   if (!valid || lane != 0) return;
@@ -358,17 +380,22 @@ __global__ void k_copy_ref(const double* __restrict__ pos, double* __restrict__ 
   if (k == 0 && counter) counter[0] += 1;
 }
 
+// the orthorhombic variants only without the virial (the engine's kernels)
 template <int KIND, int TPA, bool LANGEVIN>
 static void launch_fused_v(bool vir, hipStream_t st, int nblk, const FusedArgs& a) {
   const bool multi = a.ns > 1;
   if (vir && multi)
-    k_md_fused<KIND, TPA, LANGEVIN, true, true><<<nblk, 256, 0, st>>>(a);
+    k_md_fused<KIND, TPA, LANGEVIN, true, true, false><<<nblk, 256, 0, st>>>(a);
   else if (vir)
-    k_md_fused<KIND, TPA, LANGEVIN, true, false><<<nblk, 256, 0, st>>>(a);
+    k_md_fused<KIND, TPA, LANGEVIN, true, false, false><<<nblk, 256, 0, st>>>(a);
+  else if (multi && a.ortho)
+    k_md_fused<KIND, TPA, LANGEVIN, false, true, true><<<nblk, 256, 0, st>>>(a);
   else if (multi)
-    k_md_fused<KIND, TPA, LANGEVIN, false, true><<<nblk, 256, 0, st>>>(a);
+    k_md_fused<KIND, TPA, LANGEVIN, false, true, false><<<nblk, 256, 0, st>>>(a);
+  else if (a.ortho)
+    k_md_fused<KIND, TPA, LANGEVIN, false, false, true><<<nblk, 256, 0, st>>>(a);
   else
-    k_md_fused<KIND, TPA, LANGEVIN, false, false><<<nblk, 256, 0, st>>>(a);
+    k_md_fused<KIND, TPA, LANGEVIN, false, false, false><<<nblk, 256, 0, st>>>(a);
 }
 
 template <int KIND, int TPA>
@@ -442,7 +469,8 @@ int tsx_md_fused_step(int32_t kind, const double* params, int32_t n_species,
                       const double* kT, const double* gamma, const double* noise,
                       uint64_t seed, const int64_t* step, int32_t end_kick, int32_t do_next,
                       double half_skin, int32_t* flag_set, const float* rel_in, float* rel_out,
-                      const double* anchor, const double* ustep, void* stream) {
+                      const double* anchor, const double* ustep, int32_t orthorhombic,
+                      void* stream) {
   (void)cell_col;  // the frame (anchor, ustep) carries the geometry
   TSX_REQUIRE(kind >= TSX_LJ && kind <= TSX_SOFT_SPHERE, "unknown pair kind %d", kind);
   TSX_REQUIRE(n_species >= 1 && n_species <= MAX_SPECIES_MD, "n_species out of range");
@@ -487,6 +515,7 @@ int tsx_md_fused_step(int32_t kind, const double* params, int32_t n_species,
   a.rel_out = rel_out;
   a.anchor = anchor;
   a.ustep = ustep;
+  a.ortho = orthorhombic != 0;
   hipStream_t st = (hipStream_t)stream;
   const int nblk = blocks_for(n_atoms * tpa, 256);
   const bool vir = w_atom != nullptr;

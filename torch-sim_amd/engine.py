@@ -94,6 +94,10 @@ class BatchedMD:
         self.sysidx = SystemIndex(state.system_idx, s)
         self.sysidx.reduce_ws  # allocated (zeroed) before any graph capture
         self.cell = state.cell.detach().to(torch.float64).contiguous()
+        # every cell orthorhombic (off-diagonal entries exactly zero): the
+        # fused kernel's stencil shift then takes one product per component
+        diag = torch.diag_embed(torch.diagonal(self.cell, dim1=-2, dim2=-1))
+        self.orthorhombic = bool((self.cell == diag).all())
         pbc = state.pbc
         self.pbc_per_system = pbc.ndim == 2
         self.pbc = pbc.to(torch.uint8).contiguous()
@@ -157,7 +161,7 @@ class BatchedMD:
             int(self.langevin), ptr(self.kT), ptr(self.gamma), ptr(noise), self.seed,
             ptr(self.step_ctr), int(end_kick), int(do_next), 0.5 * self.skin,
             ptr(self._gate(1 - q)), ptr(self.rel[q]), ptr(self.rel[1 - q]), ptr(self.anchor),
-            ptr(self.ustep), stream(),
+            ptr(self.ustep), int(self.orthorhombic), stream(),
         )
 
     def _reduce(self, q: int) -> None:
