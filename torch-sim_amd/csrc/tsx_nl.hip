@@ -373,12 +373,13 @@ __device__ __forceinline__ bool gated_off(const int32_t* __restrict__ gate, int 
 // One workgroup: act[s] = number of bins of the gated-on systems before s,
 // act[n_sys] = their total.  The per-bin fill is a persistent grid that
 // walks this compact list of active bins.
-__global__ __launch_bounds__(1024) void k_active(const int32_t* __restrict__ grid,
-                                                 const int32_t* __restrict__ meta, int bin_cap,
-                                                 int n_sys, const int32_t* __restrict__ gate,
-                                                 int32_t* __restrict__ act) {
-  __shared__ int32_t sh[1024];
-  __shared__ int32_t carry;
+__device__ __forceinline__ void active_scan(const int32_t* __restrict__ grid,
+                                            const int32_t* __restrict__ meta, int bin_cap,
+                                            int n_sys, const int32_t* __restrict__ gate,
+                                            int32_t* __restrict__ act, int32_t* sh,
+                                            int32_t& carry) {
+  // 1024 threads: wave scans, then the 16 wave totals (sh[0..15])
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (threadIdx.x == 0) carry = 0;
   __syncthreads();
   const bool ok = meta[0] <= bin_cap;
@@ -389,20 +390,28 @@ __global__ __launch_bounds__(1024) void k_active(const int32_t* __restrict__ gri
       const int32_t* gr = grid + (size_t)GRID * s;
       v = gr[0] * gr[1] * gr[2];
     }
-    sh[threadIdx.x] = v;
+    const int incl = wave_incl_scan(v);
+    if (lane == 63) sh[wid] = incl;
     __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {
-      const int t = threadIdx.x >= (unsigned)d ? sh[threadIdx.x - d] : 0;
-      __syncthreads();
-      sh[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (s < n_sys) act[s] = carry + sh[threadIdx.x] - v;
-    __syncthreads();
-    if (threadIdx.x == 1023) carry += sh[1023];
+    int excl = carry + incl - v;
+    for (int q = 0; q < wid; ++q) excl += sh[q];
+    if (s < n_sys) act[s] = excl;
+    __syncthreads();  // every thread has read carry and sh
+    if (threadIdx.x == 1023) carry = excl + v;
     __syncthreads();
   }
   if (threadIdx.x == 0) act[n_sys] = carry;
+}
+
+// the scan above as a launch of its own (one 1024-thread workgroup); the
+// one-workgroup-per-system build runs it in its first workgroup instead
+__global__ __launch_bounds__(1024) void k_active(const int32_t* __restrict__ grid,
+                                                 const int32_t* __restrict__ meta, int bin_cap,
+                                                 int n_sys, const int32_t* __restrict__ gate,
+                                                 int32_t* __restrict__ act) {
+  __shared__ int32_t sh[1024];
+  __shared__ int32_t carry;
+  active_scan(grid, meta, bin_cap, n_sys, gate, act, sh, carry);
 }
 
 // system owning active work item w: last s with act[s] <= w (act is
@@ -782,7 +791,9 @@ __global__ __launch_bounds__(SB_THREADS) void k_build_small(
     double* __restrict__ xref, int32_t* __restrict__ counter,
     const double* __restrict__ cell_col, double* __restrict__ cell_build, int jbits,
     uint32_t* __restrict__ cw, int32_t* __restrict__ wmax, int stencil,
-    double* __restrict__ anchor, float* __restrict__ rel, int32_t* __restrict__ status) {
+    double* __restrict__ anchor, float* __restrict__ rel, int32_t* __restrict__ status,
+    int32_t* __restrict__ act) {
+  static_assert(SB_THREADS == 1024, "the active-bin scan needs 1024 threads");
   __shared__ int32_t lcnt[SB_MAXB + 1];  // bin counts, then exclusive starts
   __shared__ int32_t abin[SB_MAXN];      // local bin of each atom
   __shared__ int32_t aslot[SB_MAXN];     // slot of each atom in its bin
@@ -791,6 +802,8 @@ __global__ __launch_bounds__(SB_THREADS) void k_build_small(
   __shared__ int32_t carry;
   __shared__ int32_t s_wm;
   const int s = blockIdx.x;
+  // the per-bin fill's active-bin prefix (gates and grids are final already)
+  if (act && s == 0) active_scan(grid, meta, bin_cap, n_sys, gate, act, lcnt, carry);
   if (s >= n_sys || meta[0] > bin_cap || gated_off(gate, s)) return;
   const int32_t* gr = grid + (size_t)GRID * s;
   const double* g = geom + (size_t)GEOM * s;
@@ -1481,13 +1494,20 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
     TSX_CHECK_LAUNCH();
   }
   if (n_atoms == 0) return TSX_OK;
-  if (max_system_atoms > 0 && max_system_atoms <= SB_MAXN) {
+  // Fill kernel: per-bin LDS staging pays off for dense lists of many
+  // systems, where gated rebuilds touch a few percent of the atoms and one
+  // workgroup per bin shares the staged candidates among the bin's atoms; the
+  // per-atom kernel wins for sparse bins and for single large systems.
+  bool per_bin = fill_mode == TSX_FILL_BIN;
+  if (fill_mode == TSX_FILL_AUTO) per_bin = bins_per_rlist >= 2 && kmax >= 64 && n_systems > 1;
+  const bool small = max_system_atoms > 0 && max_system_atoms <= SB_MAXN;
+  if (small) {
     // every system fits one workgroup: bin, scan, scatter and sort in one launch
     k_build_small<<<n_systems, SB_THREADS, 0, st>>>(
         pos, pos_stride, sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, w.bin_count,
         w.atom_bin, w.bin_start, order, w.wr, gate, xref, counter, cell_col, cell_build, jbits,
         w.cw, w.wmax, word_mode, frame ? frame->anchor : nullptr, frame ? frame->rel : nullptr,
-        status);
+        status, per_bin ? w.act : nullptr);
     TSX_CHECK_LAUNCH();
   } else {
     // blocks per system: whole-grid parallelism for one large system, a few
@@ -1527,17 +1547,13 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
     TSX_CHECK_LAUNCH();
   }
   const double rl2 = r_list * r_list;
-  // Fill kernel: per-bin LDS staging pays off for dense lists of many
-  // systems, where gated rebuilds touch a few percent of the atoms and one
-  // wave per bin shares the staged candidates among the bin's atoms; the
-  // per-atom kernel wins for sparse bins and for single large systems.
-  bool per_bin = fill_mode == TSX_FILL_BIN;
-  if (fill_mode == TSX_FILL_AUTO) per_bin = bins_per_rlist >= 2 && kmax >= 64 && n_systems > 1;
   int lt = 0;
   while ((1 << lt) < tpa) ++lt;
   if (per_bin) {
-    k_active<<<1, 1024, 0, st>>>(w.grid, w.meta, bin_cap, n_systems, gate, w.act);
-    TSX_CHECK_LAUNCH();
+    if (!small) {  // the one-workgroup-per-system build wrote act already
+      k_active<<<1, 1024, 0, st>>>(w.grid, w.meta, bin_cap, n_systems, gate, w.act);
+      TSX_CHECK_LAUNCH();
+    }
     const int g_res = half ? fill_resident_blocks<true>() : fill_resident_blocks<false>();
     const int g_fill = bin_cap < g_res ? bin_cap : g_res;
     if (half)
