@@ -56,6 +56,7 @@ struct NLWs {
   int32_t* scan_tmp;   // scan block sums
   int32_t* unsorted;   // N: atoms scattered to their bins, before the in-bin sort
   int32_t* act;        // [S + 1] exclusive prefix of the bins of gated-on systems
+  int2* witem;         // [bin_cap] (system, bin) of each active-bin work item
   uint32_t* cw;        // N, bin order: partial list word (local index, negated wrap)
   int32_t* wmax;       // S: largest wrap-shift magnitude of the system's atoms
 };
@@ -81,6 +82,7 @@ static size_t ws_layout(int64_t n, int32_t s, int32_t bin_cap, NLWs* w, char* ba
   void* p_st = take(sizeof(int32_t) * 4096);
   void* p_un = take(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
   void* p_act = take(sizeof(int32_t) * ((size_t)(s > 0 ? s : 1) + 1));
+  void* p_wi = take(sizeof(int2) * (size_t)(bin_cap > 0 ? bin_cap : 1));
   void* p_cw = take(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
   void* p_wm = take(sizeof(int32_t) * (size_t)(s > 0 ? s : 1));
   if (w) {
@@ -95,6 +97,7 @@ static size_t ws_layout(int64_t n, int32_t s, int32_t bin_cap, NLWs* w, char* ba
     w->scan_tmp = (int32_t*)p_st;
     w->unsorted = (int32_t*)p_un;
     w->act = (int32_t*)p_act;
+    w->witem = (int2*)p_wi;
     w->cw = (uint32_t*)p_cw;
     w->wmax = (int32_t*)p_wm;
   }
@@ -792,7 +795,7 @@ __global__ __launch_bounds__(SB_THREADS) void k_build_small(
     const double* __restrict__ cell_col, double* __restrict__ cell_build, int jbits,
     uint32_t* __restrict__ cw, int32_t* __restrict__ wmax, int stencil,
     double* __restrict__ anchor, float* __restrict__ rel, int32_t* __restrict__ status,
-    int32_t* __restrict__ act) {
+    int32_t* __restrict__ act, int2* __restrict__ witem) {
   static_assert(SB_THREADS == 1024, "the active-bin scan needs 1024 threads");
   __shared__ int32_t lcnt[SB_MAXB + 1];  // bin counts, then exclusive starts
   __shared__ int32_t abin[SB_MAXN];      // local bin of each atom
@@ -808,6 +811,23 @@ __global__ __launch_bounds__(SB_THREADS) void k_build_small(
   const int32_t* gr = grid + (size_t)GRID * s;
   const double* g = geom + (size_t)GEOM * s;
   const int base = sys_start[s], n_s = sys_start[s + 1] - base;
+  if (witem) {
+    // this system's work items of the per-bin fill, written before any
+    // early exit so that [0, act[n_sys]) has no holes: its active-bin prefix
+    // (the bins of the gated-on systems before it), then (s, bin) per bin
+    int v = 0;
+    for (int k = threadIdx.x; k < s; k += SB_THREADS)
+      if (!gated_off(gate, k))
+        v += grid[(size_t)GRID * k] * grid[(size_t)GRID * k + 1] * grid[(size_t)GRID * k + 2];
+    v = wave_sum_int(v);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int first = 0;
+    for (int q = 0; q < SB_THREADS / 64; ++q) first += wsum[q];
+    const int nb0 = gr[0] * gr[1] * gr[2];
+    for (int b = threadIdx.x; b < nb0; b += SB_THREADS) witem[first + b] = make_int2(s, gr[6] + b);
+    __syncthreads();  // wsum is reused below
+  }
   if (n_s > SB_MAXN) {  // max_system_atoms was wrong: refuse, never overrun LDS
     if (threadIdx.x == 0) atomicOr(status, TSX_STATUS_BAD_SIZE);
     return;
@@ -1006,7 +1026,8 @@ __global__ __launch_bounds__(FILL_T) void k_fill(
     const int32_t* __restrict__ order, const double* __restrict__ wr,
     const uint32_t* __restrict__ cw, const int32_t* __restrict__ wmax, double rl2, int self_int,
     int jbits, int lt, int kmax, int64_t ld, uint32_t* __restrict__ nbr,
-    int32_t* __restrict__ count, int32_t* __restrict__ status, int stencil) {
+    int32_t* __restrict__ count, int32_t* __restrict__ status, int stencil,
+    const int2* __restrict__ witem) {
   __shared__ double4 cpos[FILL_CH];
   __shared__ uint32_t ccw[FILL_CH];
   __shared__ int16_t cimg[FILL_CH];
@@ -1023,14 +1044,21 @@ __global__ __launch_bounds__(FILL_T) void k_fill(
   const int tmask = (1 << lt) - 1;
   const int total = act[n_sys];
   for (int wi_ = xcd_block(); wi_ < total; wi_ += gridDim.x) {
-    const int s = active_system(act, n_sys, wi_);
+    int s, b;
+    if (witem) {  // written by the build: one load
+      const int2 it = witem[wi_];
+      s = it.x;
+      b = it.y;
+    } else {
+      s = active_system(act, n_sys, wi_);
+      b = grid[(size_t)GRID * s + 6] + (wi_ - act[s]);
+    }
     const double* g = geom + (size_t)GEOM * s;
     const int32_t* gr = grid + (size_t)GRID * s;
     const int base = sys_start[s];
     const int off = gr[6];
     const int wx = 2 * gr[3] + 1, wy = 2 * gr[4] + 1, wz = 2 * gr[5] + 1;
     const int n_img = wx * wy * wz;
-    const int b = off + (wi_ - act[s]);
     const int c0 = bin_start[b], c1 = bin_start[b + 1];
     const int nc = c1 - c0;
     if (nc == 0) continue;  // uniform over the block
@@ -1507,7 +1535,7 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
         pos, pos_stride, sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, w.bin_count,
         w.atom_bin, w.bin_start, order, w.wr, gate, xref, counter, cell_col, cell_build, jbits,
         w.cw, w.wmax, word_mode, frame ? frame->anchor : nullptr, frame ? frame->rel : nullptr,
-        status, per_bin ? w.act : nullptr);
+        status, per_bin ? w.act : nullptr, per_bin ? w.witem : nullptr);
     TSX_CHECK_LAUNCH();
   } else {
     // blocks per system: whole-grid parallelism for one large system, a few
@@ -1560,12 +1588,12 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
       k_fill<true><<<g_fill, FILL_T, 0, st>>>(
           sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, w.act, w.bin_start, order, w.wr,
           w.cw, w.wmax, rl2, self_interaction, jbits, lt, kmax, ld, nbr, count, status,
-          word_mode);
+          word_mode, small ? w.witem : nullptr);
     else
       k_fill<false><<<g_fill, FILL_T, 0, st>>>(
           sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, w.act, w.bin_start, order, w.wr,
           w.cw, w.wmax, rl2, self_interaction, jbits, lt, kmax, ld, nbr, count, status,
-          word_mode);
+          word_mode, small ? w.witem : nullptr);
   } else {
     // lanes per atom from the row capacity (dense lists: more candidates per
     // column); fewer lanes leave more atoms per wave for the column arithmetic
