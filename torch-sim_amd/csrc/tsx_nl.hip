@@ -1080,31 +1080,37 @@ __global__ __launch_bounds__(FILL_T) void k_fill(
     for (int r0 = c0; r0 < c1; r0 += per) {  // rounds of centre atoms
       const int t = r0 + slot_a;
       const bool own = t < c1;
+      // centre loads: first used by the tests, so their latency overlaps
+      // the image ranges' loads, the scan and the staging
       double4 wi = make_double4(0.0, 0.0, 0.0, 0.0);
-      int i = -1, w_i[3] = {0, 0, 0};
+      int i = -1;
       if (own) {
         wi = reinterpret_cast<const double4*>(wr)[t];
         i = order[t];
-        unpack_w(wi.w, w_i);
       }
-      const int il = i - base;
-      const uint32_t wiw = pack_fields(w_i[0], w_i[1], w_i[2], wc.sbits) << jbits;
       const int64_t rowbase = (int64_t)t << lt;
       int cnt = 0;
       bool shift_ovf = false;
       for (int ib0 = 0; ib0 < n_img; ib0 += FILL_IMG) {
         const int nib = min(FILL_IMG, n_img - ib0);
-        __syncthreads();  // previous users of the image and candidate tables are done
-        int n_q = 0;
-        if (tid < nib) {  // image tid of this batch
+        // image tid of this batch: its bin range is loaded before the barrier
+        int n_q = 0, kb = 0, dx = 0, dy = 0, dz = 0, sx = 0, sy = 0, sz = 0;
+        if (tid < nib) {
           const int im = ib0 + tid;
-          const int dz = im % wz - gr[5];
-          const int dy = (im / wz) % wy - gr[4];
-          const int dx = im / (wy * wz) - gr[3];
+          dz = im % wz - gr[5];
+          dy = (im / wz) % wy - gr[4];
+          dx = im / (wy * wz) - gr[3];
           const int cx = bx0 + dx, cy = by0 + dy, cz = bz0 + dz;
-          const int sx = floordiv(cx, gr[0]), sy = floordiv(cy, gr[1]), sz = floordiv(cz, gr[2]);
+          sx = floordiv(cx, gr[0]);
+          sy = floordiv(cy, gr[1]);
+          sz = floordiv(cz, gr[2]);
           const int nbin = off + ((cx - sx * gr[0]) * gr[1] + (cy - sy * gr[1])) * gr[2] +
                            (cz - sz * gr[2]);
+          kb = bin_start[nbin];
+          n_q = bin_start[nbin + 1] - kb;
+        }
+        __syncthreads();  // previous users of the image and candidate tables are done
+        if (tid < nib) {
           // S . L (transforms.py:563-565)
           ics[tid][0] = shift_comp(sx, sy, sz, g, 0);
           ics[tid][1] = shift_comp(sx, sy, sz, g, 1);
@@ -1114,9 +1120,7 @@ __global__ __launch_bounds__(FILL_T) void k_fill(
           isv[tid][2] = sz;
           sw[tid] = (stencil ? pack_fields(dx, dy, dz, wc.sbits) : pack_fields(sx, sy, sz, wc.sbits))
                     << jbits;
-          const int kb = bin_start[nbin], ke = bin_start[nbin + 1];
           ibeg[tid] = kb;
-          n_q = ke - kb;
         }
         // exclusive prefix of the image sizes over the block
         const int incl = wave_incl_scan_bits(n_q);
@@ -1147,6 +1151,11 @@ __global__ __launch_bounds__(FILL_T) void k_fill(
             cimg[c] = (int16_t)lo;
           }
           __syncthreads();
+          // the centre's word keys (its loads were issued with the round)
+          int w_i[3] = {0, 0, 0};
+          if (own) unpack_w(wi.w, w_i);
+          const int il = i - base;
+          const uint32_t wiw = pack_fields(w_i[0], w_i[1], w_i[2], wc.sbits) << jbits;
           for (int cb = 0; cb < nk; cb += L) {  // uniform trip count over the block
             const int c = cb + sub;
             bool match = false;
