@@ -177,6 +177,29 @@ def test_engine_batch_equals_single_bitwise_with_rebuilds(dev):
     assert total == eng.n_rebuilds
 
 
+def test_engine_orthorhombic_path_equals_general_bitwise(dev):
+    """Cubic cells take the fused kernel's orthorhombic stencil shift (one
+    product per component); forcing the general three-term form must give the
+    same bits, rebuilds included, graph replay on."""
+    st = tsa.workloads.cu_batch(3, device=dev, seed=27)
+    model = tsa.MorseModel(**MORSE, cutoff=6.0, device=dev)
+    st.rng = 5
+    kT = 1500.0 * tsa.MetalUnits.temperature
+    md = tsa.nvt_langevin_init(state=st, model=model, kT=kT)
+    dt = 0.002 * tsa.MetalUnits.time
+    runs = []
+    for ortho in (True, False):
+        eng = tsa.BatchedMD(model, md, dt=dt, kT=kT, langevin=True, skin=0.2, seed=7)
+        assert eng.orthorhombic  # cubic cells
+        eng.orthorhombic = ortho
+        eng.run(60)
+        assert eng.n_rebuilds > 0
+        runs.append(eng.state())
+    assert torch.equal(runs[0].positions, runs[1].positions)
+    assert torch.equal(runs[0].momenta, runs[1].momenta)
+    assert torch.equal(runs[0].energy, runs[1].energy)
+
+
 def test_engine_nve_energy_conservation(dev):
     st = tsa.workloads.ar_fcc(dev, reps=(4, 4, 4))
     model = tsa.LennardJonesModel(sigma=3.405, epsilon=0.0104, cutoff=8.5125, device=dev)

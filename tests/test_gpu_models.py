@@ -181,3 +181,21 @@ def test_verlet_skin_matches_rebuild_every_call(dev):
         torch.testing.assert_close(a["stress"], b["stress"], rtol=1e-12, atol=1e-12)
     # one full build at the start, more only on capacity overflow
     assert m_skin._vl.updates == 8 and m_skin._vl.full_builds <= 3
+
+
+def test_verlet_overflow_during_call_recomputes_forces(dev):
+    """The build status of a skin model is read once, after its force kernels
+    are queued.  A gated rebuild that overflows the rows (atoms pulled
+    together, same cells) must make the call rebuild every system with more
+    room and compute the forces again: they equal those of a fresh list."""
+    st = tsa.workloads.cu_batch(2, device=dev, seed=31)
+    kw = dict(sigma=2.866, epsilon=0.3429, alpha=1.3588, cutoff=6.0, device=dev)
+    m_skin = tsa.MorseModel(**kw, skin=0.5)
+    m_skin(st)
+    builds0 = m_skin._vl.full_builds
+    x = st.clone()
+    x.positions = 0.8 * st.positions  # about twice the neighbours per row
+    a, b = m_skin(x), tsa.MorseModel(**kw)(x)
+    assert m_skin._vl.full_builds > builds0  # the overflow path ran
+    torch.testing.assert_close(a["forces"], b["forces"], rtol=1e-12, atol=1e-12)
+    torch.testing.assert_close(a["energy"], b["energy"], rtol=1e-12, atol=1e-12)

@@ -824,8 +824,12 @@ __global__ __launch_bounds__(SB_THREADS) void k_build_small(
     __syncthreads();
     int first = 0;
     for (int q = 0; q < SB_THREADS / 64; ++q) first += wsum[q];
+    // a system this path refuses (too large, below) gets bin -1: the fill
+    // skips its items instead of reading bin ranges this build never wrote
     const int nb0 = gr[0] * gr[1] * gr[2];
-    for (int b = threadIdx.x; b < nb0; b += SB_THREADS) witem[first + b] = make_int2(s, gr[6] + b);
+    const bool refused = n_s > SB_MAXN;
+    for (int b = threadIdx.x; b < nb0; b += SB_THREADS)
+      witem[first + b] = make_int2(s, refused ? -1 : gr[6] + b);
     __syncthreads();  // wsum is reused below
   }
   if (n_s > SB_MAXN) {  // max_system_atoms was wrong: refuse, never overrun LDS
@@ -1049,6 +1053,7 @@ __global__ __launch_bounds__(FILL_T) void k_fill(
       const int2 it = witem[wi_];
       s = it.x;
       b = it.y;
+      if (b < 0) continue;  // a system the build refused (uniform over the block)
     } else {
       s = active_system(act, n_sys, wi_);
       b = grid[(size_t)GRID * s + 6] + (wi_ - act[s]);
