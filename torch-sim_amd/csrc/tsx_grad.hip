@@ -263,43 +263,43 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_pair_grad(
 // with respect to the shared parameter of the unordered species pair.  One
 // workgroup: thread t adds blocks t, t + 256, ... for every entry, then a
 // fixed-order tree over the workgroup (deterministic).
-constexpr int THETA_T = 256;
-constexpr int THETA_MAX = GRAD_MAX_NS * GRAD_MAX_NS * 3;
+constexpr int THETA_T = 1024;
 
+// Parameter gradient from the per-block partials: one workgroup per
+// component k (and its symmetric partner ks, the (b, a) entry), 1024
+// threads striding over the blocks, fixed-order wave and block trees, so
+// repeated calls give the same bits.  out[k] = (sum_k + sum_ks) / 2.
 __global__ __launch_bounds__(THETA_T) void k_theta_reduce(const double* __restrict__ part,
                                                           int n_blocks, int ns,
                                                           double* __restrict__ out,
                                                           int accumulate) {
   const int nb = ns * ns * 3;
-  __shared__ double red[THETA_T / 64][THETA_MAX];
-  __shared__ double tot[THETA_MAX];
-  double acc[THETA_MAX];
-#pragma unroll
-  for (int k = 0; k < THETA_MAX; ++k) acc[k] = 0.0;
-  for (int b = threadIdx.x; b < n_blocks; b += THETA_T) {
-    const double* pb = part + (int64_t)b * nb;
-#pragma unroll
-    for (int k = 0; k < THETA_MAX; ++k)
-      if (k < nb) acc[k] += pb[k];
+  const int k = blockIdx.x;
+  if (k >= nb) return;
+  const int a = k / (3 * ns), b = (k / 3) % ns, p = k % 3;
+  const int ks = (b * ns + a) * 3 + p;
+  __shared__ double red[2][THETA_T / 64];
+  double s0 = 0.0, s1 = 0.0;
+  for (int blk = threadIdx.x; blk < n_blocks; blk += THETA_T) {
+    const double* pb = part + (int64_t)blk * nb;
+    s0 += pb[k];
+    s1 += pb[ks];
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < THETA_MAX; ++k) {
-    if (k >= nb) break;
-    const double t = wave_sum(acc[k]);
-    if (lane == 0) red[wid][k] = t;
+  s0 = wave_sum(s0);
+  s1 = wave_sum(s1);
+  if (lane == 0) {
+    red[0][wid] = s0;
+    red[1][wid] = s1;
   }
   __syncthreads();
-  if (threadIdx.x < nb) {
-    double v = 0.0;
-    for (int q = 0; q < THETA_T / 64; ++q) v += red[q][threadIdx.x];
-    tot[threadIdx.x] = v;
-  }
-  __syncthreads();
-  const int k = threadIdx.x;
-  if (k < nb) {
-    const int a = k / (3 * ns), b = (k / 3) % ns, p = k % 3;
-    const double v = 0.5 * (tot[k] + tot[(b * ns + a) * 3 + p]);
+  if (threadIdx.x == 0) {
+    double t0 = 0.0, t1 = 0.0;
+    for (int q = 0; q < THETA_T / 64; ++q) {
+      t0 += red[0][q];
+      t1 += red[1][q];
+    }
+    const double v = 0.5 * (t0 + t1);
     out[k] = accumulate ? out[k] + v : v;
   }
 }
@@ -461,7 +461,7 @@ int tsx_pair_grad(int32_t kind, const double* params, int32_t n_species, const i
   TSX_REQUIRE(rc == 0, "unsupported launch configuration");
   TSX_CHECK_LAUNCH();
   if (dparams) {
-    k_theta_reduce<<<1, THETA_T, 0, st>>>(dth, nblk, ns, dparams, accumulate);
+    k_theta_reduce<<<ns * ns * 3, THETA_T, 0, st>>>(dth, nblk, ns, dparams, accumulate);
     TSX_CHECK_LAUNCH();
   }
   if (dcell_out) {
