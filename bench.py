@@ -547,7 +547,7 @@ def bench_fit(args) -> dict:
         torch.cuda.synchronize()
         return k / (time.perf_counter() - t0)
 
-    fm = timed(False, 20, 2)
+    fm = timed(False, 200, 10)  # ~0.5 ms each: enough of them to be above host jitter
     full = timed(True, args.steps, min(args.warmup, 1))
     log(f"fit: force-matching {fm:.2f} grad-evals/s, full loss ({n_md} NVE steps) {full:.3f} grad-evals/s")
     return _line("grad-evals/s at 1 MI355X, 256-system binary-LJ fit (force MSE + E after NVE)",
