@@ -1022,7 +1022,7 @@ __device__ __forceinline__ int wave_incl_scan_bits(int v) {
   return r;
 }
 
-template <bool HALF>
+template <bool HALF, bool STENCIL>
 __global__ __launch_bounds__(FILL_T) void k_fill(
     const int32_t* __restrict__ sys_start, int n_sys, const double* __restrict__ geom,
     const int32_t* __restrict__ grid, const int32_t* __restrict__ meta, int bin_cap,
@@ -1030,8 +1030,9 @@ __global__ __launch_bounds__(FILL_T) void k_fill(
     const int32_t* __restrict__ order, const double* __restrict__ wr,
     const uint32_t* __restrict__ cw, const int32_t* __restrict__ wmax, double rl2, int self_int,
     int jbits, int lt, int kmax, int64_t ld, uint32_t* __restrict__ nbr,
-    int32_t* __restrict__ count, int32_t* __restrict__ status, int stencil,
+    int32_t* __restrict__ count, int32_t* __restrict__ status,
     const int2* __restrict__ witem) {
+  constexpr bool stencil = STENCIL;  // list word format (TSX_WORDS_STENCIL)
   __shared__ double4 cpos[FILL_CH];
   __shared__ uint32_t ccw[FILL_CH];
   __shared__ int16_t cimg[FILL_CH];
@@ -1351,7 +1352,7 @@ void k_fill_atom(
 // Workgroups of the per-bin fill that fit on the device at once: its grid is
 // persistent, so a larger grid only adds a second, serial round of
 // workgroups.  Queried once per device and HALF variant.
-template <bool HALF>
+template <bool HALF, bool STENCIL>
 static int fill_resident_blocks() {
   static int cache[64] = {0};
   int dev = 0;
@@ -1359,7 +1360,8 @@ static int fill_resident_blocks() {
   if (cache[dev] > 0) return cache[dev];
   int cus = 0, per = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_fill<HALF>, FILL_T, 0) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_fill<HALF, STENCIL>, FILL_T, 0) !=
+          hipSuccess ||
       cus <= 0 || per <= 0)
     return 1024;
   cache[dev] = cus * per;
@@ -1596,18 +1598,24 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
       k_active<<<1, 1024, 0, st>>>(w.grid, w.meta, bin_cap, n_systems, gate, w.act);
       TSX_CHECK_LAUNCH();
     }
-    const int g_res = half ? fill_resident_blocks<true>() : fill_resident_blocks<false>();
+    // stencil words come with full lists only (checked above)
+    const bool sten = word_mode == TSX_WORDS_STENCIL;
+    const int g_res = half   ? fill_resident_blocks<true, false>()
+                      : sten ? fill_resident_blocks<false, true>()
+                             : fill_resident_blocks<false, false>();
     const int g_fill = bin_cap < g_res ? bin_cap : g_res;
+#define TSX_FILL_BIN_LAUNCH(HH, SS)                                                               \
+  k_fill<HH, SS><<<g_fill, FILL_T, 0, st>>>(sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, \
+                                            w.act, w.bin_start, order, w.wr, w.cw, w.wmax, rl2,    \
+                                            self_interaction, jbits, lt, kmax, ld, nbr, count,     \
+                                            status, small ? w.witem : nullptr)
     if (half)
-      k_fill<true><<<g_fill, FILL_T, 0, st>>>(
-          sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, w.act, w.bin_start, order, w.wr,
-          w.cw, w.wmax, rl2, self_interaction, jbits, lt, kmax, ld, nbr, count, status,
-          word_mode, small ? w.witem : nullptr);
+      TSX_FILL_BIN_LAUNCH(true, false);
+    else if (sten)
+      TSX_FILL_BIN_LAUNCH(false, true);
     else
-      k_fill<false><<<g_fill, FILL_T, 0, st>>>(
-          sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, w.act, w.bin_start, order, w.wr,
-          w.cw, w.wmax, rl2, self_interaction, jbits, lt, kmax, ld, nbr, count, status,
-          word_mode, small ? w.witem : nullptr);
+      TSX_FILL_BIN_LAUNCH(false, false);
+#undef TSX_FILL_BIN_LAUNCH
   } else {
     // lanes per atom from the row capacity (dense lists: more candidates per
     // column); fewer lanes leave more atoms per wave for the column arithmetic
