@@ -1224,7 +1224,7 @@ __global__ __launch_bounds__(FILL_T) void k_fill(
 // per rebuilt atom.  Four lanes per atom (dense rows, full builds of large
 // systems) run five waves per SIMD (96 VGPRs, 2-3 spilled); two lanes keep
 // the default four.
-template <int L, bool HALF>
+template <int L, bool HALF, bool STENCIL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L == 4 ? 5 : 1)))
 void k_fill_atom(
     const int32_t* __restrict__ sys_start, const int32_t* __restrict__ atom_sys,
@@ -1234,7 +1234,8 @@ void k_fill_atom(
     const double* __restrict__ wr, const uint32_t* __restrict__ cw,
     const int32_t* __restrict__ wmax, double rl2, int self_int, int jbits, int lt, int kmax,
     int64_t ld, uint32_t* __restrict__ nbr, int32_t* __restrict__ count,
-    int32_t* __restrict__ status, const int32_t* __restrict__ gate, int stencil) {
+    int32_t* __restrict__ status, const int32_t* __restrict__ gate) {
+  constexpr bool stencil = STENCIL;  // list word format (TSX_WORDS_STENCIL)
   const int64_t gid = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   const int64_t t = gid / L;
   const int sub = (int)(gid % L);
@@ -1373,9 +1374,15 @@ static void launch_fill_atom(hipStream_t st, const int32_t* sys_start, const int
                              double rl2, int self_int, int jbits, int lt, int kmax, int64_t ld,
                              uint32_t* nbr, int32_t* count, int32_t* status,
                              const int32_t* gate, int stencil) {
-  k_fill_atom<L, HALF><<<blocks_for(n * L, 256), 256, 0, st>>>(
-      sys_start, atom_sys, n, w.geom, w.grid, w.meta, bin_cap, w.atom_bin, w.bin_start, order,
-      w.wr, w.cw, w.wmax, rl2, self_int, jbits, lt, kmax, ld, nbr, count, status, gate, stencil);
+  // stencil words come with full lists only (checked by the caller)
+  if (stencil)
+    k_fill_atom<L, false, true><<<blocks_for(n * L, 256), 256, 0, st>>>(
+        sys_start, atom_sys, n, w.geom, w.grid, w.meta, bin_cap, w.atom_bin, w.bin_start, order,
+        w.wr, w.cw, w.wmax, rl2, self_int, jbits, lt, kmax, ld, nbr, count, status, gate);
+  else
+    k_fill_atom<L, HALF, false><<<blocks_for(n * L, 256), 256, 0, st>>>(
+        sys_start, atom_sys, n, w.geom, w.grid, w.meta, bin_cap, w.atom_bin, w.bin_start, order,
+        w.wr, w.cw, w.wmax, rl2, self_int, jbits, lt, kmax, ld, nbr, count, status, gate);
 }
 
 __global__ void k_emit(const uint32_t* __restrict__ nbr, const int32_t* __restrict__ count,
