@@ -1048,28 +1048,55 @@ __global__ __launch_bounds__(FILL_T) void k_fill(
   const int64_t stride = ld << lt;
   const int tmask = (1 << lt) - 1;
   const int total = act[n_sys];
-  for (int wi_ = xcd_block(); wi_ < total; wi_ += gridDim.x) {
-    int s, b;
-    if (witem) {  // written by the build: one load
-      const int2 it = witem[wi_];
-      s = it.x;
-      b = it.y;
-      if (b < 0) continue;  // a system the build refused (uniform over the block)
-    } else {
-      s = active_system(act, n_sys, wi_);
-      b = grid[(size_t)GRID * s + 6] + (wi_ - act[s]);
+  // Work item wi: (system, bin) from the build's table (bin -1: a system the
+  // build refused), or found by a search of the active-bin prefix.
+  auto item_of = [&](int wi) -> int2 {
+    if (witem) return witem[wi];
+    const int si = active_system(act, n_sys, wi);
+    return make_int2(si, grid[(size_t)GRID * si + 6] + (wi - act[si]));
+  };
+  // What a bin needs before its image ranges: the system's grid record, the
+  // bin's atom range, the system's first atom (and wrap bound), loaded
+  // together; the next item's entry is loaded one item ahead, so a bin waits
+  // on one round trip before its image ranges instead of a chain of them.
+  struct Rec {
+    int s, b, c0, c1, base, wm;
+    int gr[7];
+  };
+  auto load_rec = [&](int2 it) {
+    Rec r{};
+    r.s = it.x;
+    r.b = it.y;
+    if (it.y >= 0) {
+      const int32_t* gp = grid + (size_t)GRID * it.x;
+      for (int k = 0; k < 7; ++k) r.gr[k] = gp[k];
+      r.c0 = bin_start[it.y];
+      r.c1 = bin_start[it.y + 1];
+      r.base = sys_start[it.x];
+      if (!STENCIL) r.wm = wmax[it.x];
     }
+    return r;
+  };
+  const int first = xcd_block(), step = gridDim.x;
+  int2 nxt = make_int2(0, -1);
+  if (first < total) nxt = item_of(first);
+  for (int wi_ = first; wi_ < total; wi_ += step) {
+    const int2 it = nxt;
+    if (wi_ + step < total) nxt = item_of(wi_ + step);
+    const Rec cur = load_rec(it);
+    const int s = cur.s, b = cur.b;
+    if (b < 0) continue;  // a system the build refused (uniform over the block)
     const double* g = geom + (size_t)GEOM * s;
-    const int32_t* gr = grid + (size_t)GRID * s;
-    const int base = sys_start[s];
+    const int* gr = cur.gr;
+    const int base = cur.base;
     const int off = gr[6];
     const int wx = 2 * gr[3] + 1, wy = 2 * gr[4] + 1, wz = 2 * gr[5] + 1;
     const int n_img = wx * wy * wz;
-    const int c0 = bin_start[b], c1 = bin_start[b + 1];
+    const int c0 = cur.c0, c1 = cur.c1;
     const int nc = c1 - c0;
     if (nc == 0) continue;  // uniform over the block
     const bool fast = stencil ? stencil_fits(gr, wc.bias)
-                              : stencil_smax(gr) + 2 * wmax[s] < wc.bias;
+                              : stencil_smax(gr) + 2 * cur.wm < wc.bias;
     if (stencil && !fast) {  // stencil offsets wider than the word fields
       if (tid == 0) atomicOr(status, TSX_STATUS_SHIFT_OVERFLOW);
       continue;
@@ -1141,20 +1168,17 @@ __global__ __launch_bounds__(FILL_T) void k_fill(
         for (int k0 = 0; k0 < ncand; k0 += FILL_CH) {
           const int nk = min(FILL_CH, ncand - k0);
           if (k0 > 0) __syncthreads();  // the previous chunk's tests are done
+          // image of each candidate of the chunk, written by the image threads
+          if (tid < nib) {
+            const int e = min(excl + n_q, k0 + nk);
+            for (int c = max(excl, k0); c < e; ++c) cimg[c - k0] = (int16_t)tid;
+          }
+          __syncthreads();
           for (int c = tid; c < nk; c += FILL_T) {
-            const int gc = k0 + c;
-            int lo = 0, hi = nib - 1;  // image holding candidate gc
-            while (lo < hi) {
-              const int mid = (lo + hi + 1) >> 1;
-              if (ipre[mid] <= gc)
-                lo = mid;
-              else
-                hi = mid - 1;
-            }
-            const int k = ibeg[lo] + (gc - ipre[lo]);
+            const int q = cimg[c];
+            const int k = ibeg[q] + (k0 + c - ipre[q]);
             cpos[c] = reinterpret_cast<const double4*>(wr)[k];
             ccw[c] = cw[k];
-            cimg[c] = (int16_t)lo;
           }
           __syncthreads();
           // the centre's word keys (its loads were issued with the round)
