@@ -1163,17 +1163,24 @@ __global__ __launch_bounds__(FILL_T) void k_fill(
         for (int q = 0; q < wid; ++q) excl += wsum[q];
         if (tid < nib) ipre[tid] = excl;
         if (tid == FILL_T - 1) ipre[nib] = excl + n_q;
-        __syncthreads();
-        const int ncand = ipre[nib];
-        for (int k0 = 0; k0 < ncand; k0 += FILL_CH) {
-          const int nk = min(FILL_CH, ncand - k0);
-          if (k0 > 0) __syncthreads();  // the previous chunk's tests are done
-          // image of each candidate of the chunk, written by the image threads
+        // image of each candidate of a chunk, written by the image threads
+        // (the first chunk's under the same barrier as the prefix)
+        auto image_table = [&](int k0, int nk) {
           if (tid < nib) {
             const int e = min(excl + n_q, k0 + nk);
             for (int c = max(excl, k0); c < e; ++c) cimg[c - k0] = (int16_t)tid;
           }
-          __syncthreads();
+        };
+        image_table(0, FILL_CH);
+        __syncthreads();
+        const int ncand = ipre[nib];
+        for (int k0 = 0; k0 < ncand; k0 += FILL_CH) {
+          const int nk = min(FILL_CH, ncand - k0);
+          if (k0 > 0) {
+            __syncthreads();  // the previous chunk's tests are done
+            image_table(k0, nk);
+            __syncthreads();
+          }
           for (int c = tid; c < nk; c += FILL_T) {
             const int q = cimg[c];
             const int k = ibeg[q] + (k0 + c - ipre[q]);
@@ -1186,38 +1193,72 @@ __global__ __launch_bounds__(FILL_T) void k_fill(
           if (own) unpack_w(wi.w, w_i);
           const int il = i - base;
           const uint32_t wiw = pack_fields(w_i[0], w_i[1], w_i[2], wc.sbits) << jbits;
-          for (int cb = 0; cb < nk; cb += L) {  // uniform trip count over the block
-            const int c = cb + sub;
+          // candidate c of the chunk against this lane's centre atom; the
+          // distance of a clamped index is computed unconditionally so that
+          // the LDS reads of several candidates can be in flight together
+          struct Cand {
+            double4 wj;
+            int q;
+            double d2;
+          };
+          auto dist = [&](int c) -> Cand {
+            Cand r;
+            const int cc = min(c, nk - 1);
+            r.wj = cpos[cc];
+            r.q = cimg[cc];
+            r.d2 = dist2_ref(wi.x, wi.y, wi.z, r.wj.x, r.wj.y, r.wj.z, ics[r.q][0], ics[r.q][1],
+                             ics[r.q][2]);
+            return r;
+          };
+          auto test = [&](int c, const Cand& k, uint32_t& word) -> bool {
             bool match = false;
-            uint32_t word = 0;
-            if (own && c < nk) {
-              const double4 wj = cpos[c];
-              const int q = cimg[c];
-              const double d2 =
-                  dist2_ref(wi.x, wi.y, wi.z, wj.x, wj.y, wj.z, ics[q][0], ics[q][1], ics[q][2]);
-              if (d2 < rl2) {
-                const uint32_t cwj = ccw[c];
-                const int jl = (int)(cwj & wc.jmask);
-                if (fast) {
-                  const uint32_t swq = sw[q];
-                  word = stencil ? (uint32_t)jl | swq
-                                 : (uint32_t)jl |
-                                       field_add(cwj & ~wc.jmask, field_add(swq, wiw, wc.H), wc.H);
-                  match = self_int || !(swq == 0 && jl == il);
-                  if (HALF && match) match = half_keep(wc, word, il, jl);
-                } else {
-                  const int sx = isv[q][0], sy = isv[q][1], sz = isv[q][2];
-                  match = self_int || !(jl == il && (sx | sy | sz) == 0);
-                  checked_word(wc, jl, sx, sy, sz, w_i, wj.w, HALF, il, match, word, shift_ovf);
-                }
+            if (own && c < nk && k.d2 < rl2) {
+              const int q = k.q;
+              const uint32_t cwj = ccw[c];
+              const int jl = (int)(cwj & wc.jmask);
+              if (fast) {
+                const uint32_t swq = sw[q];
+                word = stencil ? (uint32_t)jl | swq
+                               : (uint32_t)jl |
+                                     field_add(cwj & ~wc.jmask, field_add(swq, wiw, wc.H), wc.H);
+                match = self_int || !(swq == 0 && jl == il);
+                if (HALF && match) match = half_keep(wc, word, il, jl);
+              } else {
+                const int sx = isv[q][0], sy = isv[q][1], sz = isv[q][2];
+                match = self_int || !(jl == il && (sx | sy | sz) == 0);
+                checked_word(wc, jl, sx, sy, sz, w_i, k.wj.w, HALF, il, match, word, shift_ovf);
               }
             }
+            return match;
+          };
+          // matches ordered by a ballot over the group, in candidate order
+          auto emit = [&](bool match, uint32_t word) {
             const uint64_t bal = __ballot(match);
             if (match) {
               const int pos_n = cnt + __popcll(bal & below);
               if (pos_n < kmax) nbr[rowbase + (int64_t)(pos_n >> lt) * stride + (pos_n & tmask)] = word;
             }
             cnt += __popcll(bal & gmask);
+          };
+          // uniform trip counts over the block
+          if (STENCIL) {
+            // two candidates per lane and trip, so the LDS reads of the second
+            // overlap the first's arithmetic (68 VGPRs; the other word formats
+            // would pass the 80 that six workgroups per CU allow)
+            for (int cb = 0; cb < nk; cb += 2 * L) {
+              const Cand k0c = dist(cb + sub), k1c = dist(cb + L + sub);
+              uint32_t w0 = 0, w1 = 0;
+              const bool m0 = test(cb + sub, k0c, w0);
+              const bool m1 = test(cb + L + sub, k1c, w1);
+              emit(m0, w0);
+              if (cb + L < nk) emit(m1, w1);
+            }
+          } else {
+            for (int cb = 0; cb < nk; cb += L) {
+              uint32_t w0 = 0;
+              const bool m0 = test(cb + sub, dist(cb + sub), w0);
+              emit(m0, w0);
+            }
           }
         }
       }
@@ -1232,7 +1273,8 @@ __global__ __launch_bounds__(FILL_T) void k_fill(
         count[t] = cnt;
       }
     }
-    __syncthreads();  // the next bin rewrites the tables
+    // the next bin's first image batch opens with a barrier before it
+    // rewrites the tables, so none is needed here
   }
 }
 
