@@ -129,12 +129,13 @@ def _raw_rows(nl, n):
     return nl.order[:n].cpu().tolist(), rows
 
 
-@pytest.mark.parametrize("q", [1, 2])
+@pytest.mark.parametrize("q", [1, 2, 3])
 def test_fill_kernels_write_identical_rows(dev, q):
     """The per-bin and per-atom fill kernels write every row with the same
     neighbour words in the same order (stencil image, then atom index), on
     small triclinic cells (many periodic images) and larger ones, full and
-    half lists."""
+    half lists.  Three bins per list radius give 343 stencil images, more
+    than one batch of the per-bin fill."""
     st = tsa.workloads.cu_batch(6, device="cpu", seed=8)
     small = tsa.workloads.cu_batch(3, device="cpu", reps=[[2, 2, 2], [1, 1, 1], [1, 2, 3]], seed=2)
     tri = tsa.workloads.soft_sphere_batch(4, n_min=40, n_max=120, device="cpu", seed=3)

@@ -307,6 +307,23 @@ void k_md_fused(FusedArgs A) {
   const RowAcc a = accumulate_rel<KIND, TPA, VIRIAL, MULTI, ORTHO>(
       sp, A.ns, ti, rel_s, (float)ex.xi, (float)ex.yi, (float)ex.zi, uf, A.nbr, tt, cnt, lane,
       A.jbits, A.ld, lo2, hi2, ex);
+  // Philox normals of (seed, step, atom) for the O step: with two or more
+  // lanes per atom, lanes 0 and 1 each draw one of normal3's two blocks (the
+  // same operations on each block, so the same values), in one instruction
+  // stream instead of two on lane 0
+  double zn[3] = {0.0, 0.0, 0.0};
+  if (LANGEVIN && TPA >= 2 && A.do_next && !A.noise) {
+    const uint64_t step = (uint64_t)A.step[0];
+    const uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);
+    const u4 r = philox4x32_10(
+        u4{(uint32_t)i, (uint32_t)step, (uint32_t)(step >> 32), (uint32_t)(lane & 1)}, k0, k1);
+    const double amp = sqrt(-2.0 * log(u01_53(r.x, r.y)));
+    double sn, cs;
+    sincospi(2.0 * u01_53(r.z, r.w), &sn, &cs);
+    zn[0] = amp * cs;
+    zn[1] = amp * sn;
+    zn[2] = __shfl_xor(zn[0], 1, 64);  // block 1's first normal, from lane 1
+  }
   if (!valid || lane != 0) return;
   // own position and anchor again (cached): not kept live across the row walk
   const double4 xi = reinterpret_cast<const double4*>(A.pos_in)[i];
@@ -331,8 +348,8 @@ void k_md_fused(FusedArgs A) {
       const double c1 = exp(-A.gamma[s] * A.dt);
       const double c2 = sqrt(A.kT[s] * (1.0 - c1 * c1));
       const double sm = sqrt(mi);
-      double z[3];
-      langevin_noise(A.noise, i, A.seed, (uint64_t)A.step[0], z);
+      double z[3] = {zn[0], zn[1], zn[2]};
+      if (A.noise || TPA == 1) langevin_noise(A.noise, i, A.seed, (uint64_t)A.step[0], z);
       p0 = c1 * p0 + c2 * sm * z[0];
       p1 = c1 * p1 + c2 * sm * z[1];
       p2 = c1 * p2 + c2 * sm * z[2];
