@@ -40,10 +40,10 @@ MORSE = dict(sigma=2.866, epsilon=0.3429, alpha=1.3588)
 # WRITE_SIZE x 1.000).  Keyed by (workload, skin, tpa); the summaries with their commands
 # are profiles/k_md_fused__cu_morse_nvt_512sys.md and profiles/k_md_fused__lj_fluid_1m_nve.md.
 MEASURED_TRAFFIC = {
-    ("cu_morse_nvt", 0.6, 2): {"fetch_kib": 111997.6, "fetch_factor": 2.000,
-                               "write_kib": 41870.5, "write_factor": 1.000},
-    ("cu_morse_nvt", 0.72, 2): {"fetch_kib": 123114.0, "fetch_factor": 2.000,
-                                "write_kib": 41864.1, "write_factor": 1.000},
+    ("cu_morse_nvt", 0.6, 2): {"fetch_kib": 112003.0, "fetch_factor": 2.000,
+                               "write_kib": 41782.5, "write_factor": 1.000},
+    ("cu_morse_nvt", 0.72, 2): {"fetch_kib": 123110.2, "fetch_factor": 2.000,
+                                "write_kib": 41766.6, "write_factor": 1.000},
     ("lj1m_nve", 0.3, 4): {"fetch_kib": 342882.3, "fetch_factor": 2.000,
                            "write_kib": 125753.6, "write_factor": 1.000},
 }
