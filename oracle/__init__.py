@@ -26,6 +26,8 @@ Modules:
     mat33   -- 3x3 expm / logm / Frechet derivative (torch_sim/math.py),
                checked against scipy.linalg.
     units   -- MetalUnits constants (torch_sim/units.py).
+    rng     -- Philox4x32-10 + Box-Muller Langevin normals of the fused engine
+               (the reference draws torch.randn, integrators/nvt.py:74-83).
 """
 
 PINNING = {
@@ -55,4 +57,8 @@ PINNING = {
     "NVE adjoint (autograd through md.nve_step)": "central finite differences of "
     "E(x_T) wrt the pair parameters; checkpoint-interval invariance",
     "mat33.*": "scipy.linalg.expm / logm / expm_frechet",
+    "rng.philox4x32_10": "Random123 known-answer vectors (counter/key all zero, all ones, "
+    "and the pi-digit pair)",
+    "rng.langevin_normals": "Box-Muller identities (|z0, z1| radius from u1, angle from u2); "
+    "moments over 2^18 draws",
 }

@@ -11,7 +11,7 @@ import pytest
 import torch
 
 from oracle import fire as ofire
-from oracle import mat33, md as omd, models as om, nl as onl
+from oracle import mat33, md as omd, models as om, nl as onl, rng as orng
 
 GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
 F64 = torch.float64
@@ -553,3 +553,44 @@ def test_nve_adjoint_oracle_matches_finite_differences():
         d[i, c] = h
         fd = (final_energy(pos + d, KA_SIG, KA_EPS, False) - final_energy(pos - d, KA_SIG, KA_EPS, False)).item() / (2 * h)
         assert abs(gx[i, c].item() - fd) <= 1e-6 * max(abs(fd), 1e-3), (i, c, gx[i, c].item(), fd)
+
+
+# ------------------------------------------------------------- Langevin noise
+@pytest.mark.parametrize("ctr,key,out", [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+])
+def test_philox_known_answers(ctr, key, out):
+    """Random123's Philox4x32-10 known-answer vectors."""
+    got = orng.philox4x32_10(ctr, key)
+    assert tuple(int(x) for x in got) == out
+
+
+def test_langevin_normals_box_muller_and_moments():
+    """z0, z1 lie on the circle of radius sqrt(-2 log u1) at angle 2 pi u2 of
+    block 0; z2 is block 1's cosine normal; about 4e5 draws have the first
+    four moments of N(0, 1) and differ between steps, seeds and components."""
+    n = 1 << 16
+    z = orng.langevin_normals(seed=12345, step=7, n_atoms=n)
+    i = np.arange(n, dtype=np.uint64)
+    r0 = orng.philox4x32_10((i, 7, 0, 0), (12345, 0))
+    r1 = orng.philox4x32_10((i, 7, 0, 1), (12345, 0))
+    rad = np.sqrt(-2.0 * np.log(orng.u01_53(r0[0], r0[1])))
+    np.testing.assert_allclose(np.hypot(z[:, 0], z[:, 1]), rad, rtol=1e-13)
+    ang = np.arctan2(z[:, 1], z[:, 0]) % (2 * np.pi)
+    d = np.abs(ang - 2 * np.pi * orng.u01_53(r0[2], r0[3]))
+    assert np.minimum(d, 2 * np.pi - d).max() < 1e-9
+    rad1 = np.sqrt(-2.0 * np.log(orng.u01_53(r1[0], r1[1])))
+    np.testing.assert_allclose(np.abs(z[:, 2]), rad1 * np.abs(np.cos(2 * np.pi * orng.u01_53(r1[2], r1[3]))),
+                               rtol=1e-12, atol=1e-15)
+    allz = np.concatenate([z, orng.langevin_normals(seed=12345, step=8, n_atoms=n)]).ravel()
+    assert abs(allz.mean()) < 0.01 and abs(allz.var() - 1.0) < 0.01
+    assert abs((allz ** 3).mean()) < 0.03 and abs((allz ** 4).mean() - 3.0) < 0.05
+    other = orng.langevin_normals(seed=12346, step=7, n_atoms=n)
+    assert abs(np.corrcoef(z[:, 0], other[:, 0])[0, 1]) < 0.02
+    assert abs(np.corrcoef(z[:, 0], z[:, 2])[0, 1]) < 0.02
+    # the step counter's high word reaches the generator
+    hi = orng.langevin_normals(seed=12345, step=7 + (1 << 32), n_atoms=8)
+    assert not np.array_equal(hi, z[:8])
