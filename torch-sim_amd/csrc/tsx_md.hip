@@ -65,13 +65,14 @@ __global__ void k_baoa(double* __restrict__ x, double* __restrict__ p,
   const double c1 = exp(-gamma[s] * dt);
   const double c2 = sqrt(kT[s] * (1.0 - c1 * c1));
   const double sm = sqrt(mi);
+  const double inv_m = 1.0 / mi;  // the fused step's drift arithmetic
   double z[3];
   langevin_noise(noise, i, seed, step, z);
   for (int c = 0; c < 3; ++c) {
     double pc = p[3 * i + c] + f[3 * i + c] * hdt;
-    double xc = x[3 * i + c] + (pc / mi) * hdt;
+    double xc = x[3 * i + c] + (pc * inv_m) * hdt;
     pc = c1 * pc + c2 * sm * z[c];
-    xc = xc + (pc / mi) * hdt;
+    xc = xc + (pc * inv_m) * hdt;
     p[3 * i + c] = pc;
     x[3 * i + c] = xc;
   }
@@ -341,10 +342,11 @@ void k_md_fused(FusedArgs A) {
     p0 = p0 + a.fx * hdt;  // B(dt/2) opening the next step
     p1 = p1 + a.fy * hdt;
     p2 = p2 + a.fz * hdt;
+    const double inv_m = 1.0 / mi;  // one division for the drifts
     if (LANGEVIN) {
-      xo.x = xo.x + (p0 / mi) * hdt;
-      xo.y = xo.y + (p1 / mi) * hdt;
-      xo.z = xo.z + (p2 / mi) * hdt;
+      xo.x = xo.x + (p0 * inv_m) * hdt;
+      xo.y = xo.y + (p1 * inv_m) * hdt;
+      xo.z = xo.z + (p2 * inv_m) * hdt;
       const double c1 = exp(-A.gamma[s] * A.dt);
       const double c2 = sqrt(A.kT[s] * (1.0 - c1 * c1));
       const double sm = sqrt(mi);
@@ -353,13 +355,13 @@ void k_md_fused(FusedArgs A) {
       p0 = c1 * p0 + c2 * sm * z[0];
       p1 = c1 * p1 + c2 * sm * z[1];
       p2 = c1 * p2 + c2 * sm * z[2];
-      xo.x = xo.x + (p0 / mi) * hdt;
-      xo.y = xo.y + (p1 / mi) * hdt;
-      xo.z = xo.z + (p2 / mi) * hdt;
+      xo.x = xo.x + (p0 * inv_m) * hdt;
+      xo.y = xo.y + (p1 * inv_m) * hdt;
+      xo.z = xo.z + (p2 * inv_m) * hdt;
     } else {
-      xo.x = xo.x + (p0 / mi) * A.dt;
-      xo.y = xo.y + (p1 / mi) * A.dt;
-      xo.z = xo.z + (p2 / mi) * A.dt;
+      xo.x = xo.x + (p0 * inv_m) * A.dt;
+      xo.y = xo.y + (p1 * inv_m) * A.dt;
+      xo.z = xo.z + (p2 * inv_m) * A.dt;
     }
     if (A.xref && A.flag_set) {
       const double4 r = reinterpret_cast<const double4*>(A.xref)[i];
