@@ -139,7 +139,8 @@ const char* tsx_last_error(void);
 #define TSX_WORDS_SHIFT 0
 #define TSX_WORDS_STENCIL 1
 
-/* Bin-relative frame written by a build (any pointer may be NULL):
+/* Bin-relative frame written by a build (any pointer may be NULL but
+ * e_atom says whether the next four are used):
  *   anchor [N,4] float64, by atom index: A_i = sum_k (w_ik + b_ik / nb_k) a_k,
  *       the corner of atom i's bin in its unwrapped image (w = wrap shift,
  *       b = bin, a_k = lattice vector k); along a non-periodic direction the
@@ -150,11 +151,23 @@ const char* tsx_last_error(void);
  *       32 bits of the packed species word of a stride-4 position;
  *   ustep [S,12] float64: entries 3k..3k+2 = a_k / nb_k, entry 9 =
  *       sum_k |a_k / nb_k| (written by setup builds only).
- * Anchor and rel are written for the rebuilt systems. */
+ * Anchor and rel are written for the rebuilt systems.
+ *
+ * e_atom [N] float64, energy [S], gate_clear [S], step [1] (optional, NULL
+ * e_atom: unused): the per-system reduction of the MD step before this
+ * rebuild, done by the build's one-workgroup-per-system kernel instead of a
+ * launch of its own.  Every system s, rebuilt or not, gets energy[s] = the
+ * sum of e_atom over its atoms in the fixed order of tsx_md_reduce (bitwise
+ * the same value), gate_clear[s] = 0, and step[0] advances by one.  Needs
+ * 0 < max_system_atoms <= 2048 and n_atoms > 0 (the one-workgroup build). */
 typedef struct tsx_nl_frame {
   double* anchor;
   float* rel;
   double* ustep;
+  const double* e_atom;
+  double* energy;
+  int32_t* gate_clear;
+  int64_t* step;
 } tsx_nl_frame;
 
 int64_t tsx_nl_ws_bytes(int64_t n_atoms, int32_t n_systems, int32_t bin_cap);

@@ -142,6 +142,33 @@ def test_engine_internal_noise_is_the_philox_stream(dev, tpa):
     assert (out[0].momenta - md.momenta).abs().max().item() > 0.01  # the noise did act
 
 
+def test_rebuild_launch_reduction_equals_own_launch(dev):
+    """With systems of at most 2048 atoms the per-system reduction of a step
+    (energies, gate clearing, step counter) runs inside the next rebuild's
+    launch: trajectories, energies and the step counter are bitwise those of
+    the engine that launches the reduction on its own, with and without graph
+    replay and across a chunk boundary."""
+    st, model, md, kT = _setup(dev, n_sys=4, seed=33)
+    dt = 0.002 * tsa.MetalUnits.time
+    for use_graph in (True, False):
+        runs = []
+        for merge in (True, False):
+            eng = tsa.BatchedMD(model, md, dt=dt, kT=kT, langevin=True, skin=0.3, seed=11,
+                                use_graph=use_graph, chunk_steps=8)
+            assert eng._merge_reduce
+            eng._merge_reduce = merge
+            print("graph", use_graph, "merged reduction", merge, flush=True)
+            eng.run(21)
+            eng.run(4)
+            runs.append((eng.state(), int(eng.step_ctr.item()), int(eng.n_rebuilds)))
+        (a, ca, ra), (b, cb, rb) = runs
+        assert ra == rb and ra > 0
+        assert ca == cb
+        assert torch.equal(a.positions, b.positions)
+        assert torch.equal(a.momenta, b.momenta)
+        assert torch.equal(a.energy, b.energy)
+
+
 def test_engine_graph_replay_equals_eager(dev):
     st, model, md, kT = _setup(dev, seed=24)
     dt = 0.002 * tsa.MetalUnits.time

@@ -108,6 +108,10 @@ class NLFrame(ctypes.Structure):
         ("anchor", ctypes.c_void_p),
         ("rel", ctypes.c_void_p),
         ("ustep", ctypes.c_void_p),
+        ("e_atom", ctypes.c_void_p),
+        ("energy", ctypes.c_void_p),
+        ("gate_clear", ctypes.c_void_p),
+        ("step", ctypes.c_void_p),
     ]
 
 

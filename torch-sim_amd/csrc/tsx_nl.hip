@@ -795,7 +795,8 @@ __global__ __launch_bounds__(SB_THREADS) void k_build_small(
     const double* __restrict__ cell_col, double* __restrict__ cell_build, int jbits,
     uint32_t* __restrict__ cw, int32_t* __restrict__ wmax, int stencil,
     double* __restrict__ anchor, float* __restrict__ rel, int32_t* __restrict__ status,
-    int32_t* __restrict__ act, int2* __restrict__ witem) {
+    int32_t* __restrict__ act, int2* __restrict__ witem, const double* __restrict__ red_e,
+    double* __restrict__ red_energy, int32_t* __restrict__ red_gate, int64_t* __restrict__ red_step) {
   static_assert(SB_THREADS == 1024, "the active-bin scan needs 1024 threads");
   __shared__ int32_t lcnt[SB_MAXB + 1];  // bin counts, then exclusive starts
   __shared__ int32_t abin[SB_MAXN];      // local bin of each atom
@@ -804,9 +805,44 @@ __global__ __launch_bounds__(SB_THREADS) void k_build_small(
   __shared__ int32_t wsum[SB_THREADS / 64];
   __shared__ int32_t carry;
   __shared__ int32_t s_wm;
+  __shared__ double red_part[4];
   const int s = blockIdx.x;
   // the per-bin fill's active-bin prefix (gates and grids are final already)
   if (act && s == 0) active_scan(grid, meta, bin_cap, n_sys, gate, act, lcnt, carry);
+  // the previous MD step's energy of system s, in tsx_md_reduce's order:
+  // threads 0-255 sum their strided atoms of the system's first 2048, each
+  // wave sums its lanes by the xor tree, thread 0 adds the four waves
+  if (red_e && s < n_sys) {
+    constexpr int RT = 256;
+    const int64_t lo = sys_start[s], end = sys_start[s + 1];
+    const int64_t hi = lo + 2048 < end ? lo + 2048 : end;
+    // the (at most eight) loads of a thread are issued together; the adds
+    // keep the strided order
+    double acc = 0.0;
+    if (threadIdx.x < RT && lo < hi) {
+      double v[2048 / RT];
+#pragma unroll
+      for (int u = 0; u < 2048 / RT; ++u) {
+        const int64_t k = lo + threadIdx.x + (int64_t)u * RT;
+        v[u] = red_e[k < hi ? k : lo];
+      }
+#pragma unroll
+      for (int u = 0; u < 2048 / RT; ++u)
+        if (lo + threadIdx.x + (int64_t)u * RT < hi) acc += v[u];
+    }
+    if (threadIdx.x < RT) {
+      const double t = wave_sum(acc);
+      if ((threadIdx.x & 63) == 0) red_part[threadIdx.x >> 6] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int w = 0; w < RT / 64; ++w) t += red_part[w];
+      red_energy[s] = t;
+      red_gate[s] = 0;
+      if (s == 0) red_step[0] += 1;
+    }
+  }
   if (s >= n_sys || meta[0] > bin_cap || gated_off(gate, s)) return;
   const int32_t* gr = grid + (size_t)GRID * s;
   const double* g = geom + (size_t)GEOM * s;
@@ -1610,6 +1646,8 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
         w.grid, w.meta, status, w.bin_count, nullptr, frame ? frame->ustep : nullptr);
     TSX_CHECK_LAUNCH();
   }
+  TSX_REQUIRE(!(frame && frame->e_atom) || n_atoms > 0,
+              "the build's per-system reduction needs atoms");
   if (n_atoms == 0) return TSX_OK;
   // Fill kernel: per-bin LDS staging pays off for dense lists of many
   // systems, where gated rebuilds touch a few percent of the atoms and one
@@ -1618,13 +1656,18 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
   bool per_bin = fill_mode == TSX_FILL_BIN;
   if (fill_mode == TSX_FILL_AUTO) per_bin = bins_per_rlist >= 2 && kmax >= 64 && n_systems > 1;
   const bool small = max_system_atoms > 0 && max_system_atoms <= SB_MAXN;
+  const bool red = frame && frame->e_atom;
+  TSX_REQUIRE(!red || small, "the build's per-system reduction needs max_system_atoms in (0, 2048]");
+  TSX_REQUIRE(!red || (frame->energy && frame->gate_clear && frame->step),
+              "the build's per-system reduction needs energy, gate_clear and step");
   if (small) {
     // every system fits one workgroup: bin, scan, scatter and sort in one launch
     k_build_small<<<n_systems, SB_THREADS, 0, st>>>(
         pos, pos_stride, sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, w.bin_count,
         w.atom_bin, w.bin_start, order, w.wr, gate, xref, counter, cell_col, cell_build, jbits,
         w.cw, w.wmax, word_mode, frame ? frame->anchor : nullptr, frame ? frame->rel : nullptr,
-        status, per_bin ? w.act : nullptr, per_bin ? w.witem : nullptr);
+        status, per_bin ? w.act : nullptr, per_bin ? w.witem : nullptr, red ? frame->e_atom : nullptr,
+        red ? frame->energy : nullptr, red ? frame->gate_clear : nullptr, red ? frame->step : nullptr);
     TSX_CHECK_LAUNCH();
   } else {
     // blocks per system: whole-grid parallelism for one large system, a few

@@ -68,10 +68,25 @@ __global__ __launch_bounds__(RED_THREADS) void k_seg_reduce(
   const int64_t hi = lo + RED_CH < end ? lo + RED_CH : end;
   double acc[V];
   for (int q = 0; q < V; ++q) acc[q] = 0.0;
-  for (int64_t k = lo + threadIdx.x; k < hi; k += RED_THREADS) {
-    double v[V];
-    load_vals<V>(e_atom, w_atom, k, v);
-    for (int q = 0; q < V; ++q) acc[q] += v[q];
+  if (V == 1 && lo < hi) {
+    // the (at most RED_CH / RED_THREADS) loads of a thread are issued
+    // together; the adds keep the strided order of the loop below
+    constexpr int PT = RED_CH / RED_THREADS;
+    double v[PT];
+#pragma unroll
+    for (int u = 0; u < PT; ++u) {
+      const int64_t k = lo + threadIdx.x + (int64_t)u * RED_THREADS;
+      v[u] = e_atom[k < hi ? k : lo];
+    }
+#pragma unroll
+    for (int u = 0; u < PT; ++u)
+      if (lo + threadIdx.x + (int64_t)u * RED_THREADS < hi) acc[0] += v[u];
+  } else {
+    for (int64_t k = lo + threadIdx.x; k < hi; k += RED_THREADS) {
+      double v[V];
+      load_vals<V>(e_atom, w_atom, k, v);
+      for (int q = 0; q < V; ++q) acc[q] += v[q];
+    }
   }
   for (int q = 0; q < V; ++q) {
     const double t = wave_sum(acc[q]);

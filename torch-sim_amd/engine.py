@@ -138,6 +138,17 @@ class BatchedMD:
             _lib.NLFrame(self.anchor.data_ptr(), self.rel[q].data_ptr(), self.ustep.data_ptr())
             for q in range(2)
         ]
+        # With systems of at most 2048 atoms the per-system reduction of a step
+        # (energies, clearing the gate row its rebuild read, the step counter)
+        # runs inside the next step's rebuild launch; the last step of a run
+        # and the other layouts keep the reduction launch of their own.
+        self._merge_reduce = self.nl.one_block_build and 0 < self.sysidx.max_atoms <= 2048
+        self._frames_red = [
+            _lib.NLFrame(self.anchor.data_ptr(), self.rel[q].data_ptr(), self.ustep.data_ptr(),
+                         self.e_atom.data_ptr(), self.energy.data_ptr(),
+                         self._gate(1 - q).data_ptr(), self.step_ctr.data_ptr())
+            for q in range(2)
+        ]
         self.cur = 0
         self.steps_done = 0
         self._graph = None
@@ -177,13 +188,14 @@ class BatchedMD:
         self.nl.enqueue(
             self.pos[q], 4, self.cell, self.pbc, self.pbc_per_system, self.sysidx,
             setup=False, gate=self._gate(q), xref=self.xref, counter=self.rebuilds,
-            frame=self._frames[q],
+            frame=self._frames_red[q] if self._merge_reduce else self._frames[q],
         )
 
     def _step(self, q: int, do_next: bool, noise=None) -> None:
         self._rebuild(q)
         self._fused(q, end_kick=True, do_next=do_next, noise=noise)
-        self._reduce(q)
+        if not (self._merge_reduce and do_next):  # else the next rebuild reduces
+            self._reduce(q)
 
     # ------------------------------------------------------------ control
     def initialize(self) -> None:
@@ -202,7 +214,8 @@ class BatchedMD:
         step boundary) and the opening half of the next step."""
         q = self.cur
         self._fused(q, end_kick=False, do_next=True, noise=self._next_noise())
-        self._reduce(q)
+        if not self._merge_reduce:  # else the next rebuild reduces
+            self._reduce(q)
         self.cur = 1 - q
 
     def _next_noise(self):
