@@ -155,8 +155,8 @@ const char* tsx_last_error(void);
  *
  * e_atom [N] float64, energy [S], gate_clear [S], step [1] (optional, NULL
  * e_atom: unused): the per-system reduction of the MD step before this
- * rebuild, done by the build's one-workgroup-per-system kernel instead of a
- * launch of its own.  Every system s, rebuilt or not, gets energy[s] = the
+ * rebuild, done by extra workgroups of the build's one-workgroup-per-system
+ * kernel (beside the builds) instead of a launch of its own.  Every system s, rebuilt or not, gets energy[s] = the
  * sum of e_atom over its atoms in the fixed order of tsx_md_reduce (bitwise
  * the same value), gate_clear[s] = 0, and step[0] advances by one.  Needs
  * 0 < max_system_atoms <= 2048 and n_atoms > 0 (the one-workgroup build). */

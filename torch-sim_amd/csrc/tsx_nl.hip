@@ -805,43 +805,48 @@ __global__ __launch_bounds__(SB_THREADS) void k_build_small(
   __shared__ int32_t wsum[SB_THREADS / 64];
   __shared__ int32_t carry;
   __shared__ int32_t s_wm;
-  __shared__ double red_part[4];
+  __shared__ double red_part[SB_THREADS / 256][4];
   const int s = blockIdx.x;
   // the per-bin fill's active-bin prefix (gates and grids are final already)
   if (act && s == 0) active_scan(grid, meta, bin_cap, n_sys, gate, act, lcnt, carry);
-  // the previous MD step's energy of system s, in tsx_md_reduce's order:
-  // threads 0-255 sum their strided atoms of the system's first 2048, each
-  // wave sums its lanes by the xor tree, thread 0 adds the four waves
-  if (red_e && s < n_sys) {
+  // Workgroups past the systems reduce the previous MD step's energies, four
+  // systems each, 256 threads per system in tsx_md_reduce's order: a thread
+  // sums its strided atoms of the system's first 2048, each wave sums its
+  // lanes by the xor tree, the group's first thread adds its four waves.
+  if (s >= n_sys) {
+    if (!red_e) return;
     constexpr int RT = 256;
-    const int64_t lo = sys_start[s], end = sys_start[s + 1];
-    const int64_t hi = lo + 2048 < end ? lo + 2048 : end;
-    // the (at most eight) loads of a thread are issued together; the adds
-    // keep the strided order
-    double acc = 0.0;
-    if (threadIdx.x < RT && lo < hi) {
-      double v[2048 / RT];
+    const int g = threadIdx.x / RT, tg = threadIdx.x % RT;
+    const int sr = (s - n_sys) * (SB_THREADS / RT) + g;
+    if (sr < n_sys) {
+      const int64_t lo = sys_start[sr], end = sys_start[sr + 1];
+      const int64_t hi = lo + 2048 < end ? lo + 2048 : end;
+      double acc = 0.0;
+      if (lo < hi) {
+        // the (at most eight) loads of a thread are issued together; the
+        // adds keep the strided order
+        double v[2048 / RT];
 #pragma unroll
-      for (int u = 0; u < 2048 / RT; ++u) {
-        const int64_t k = lo + threadIdx.x + (int64_t)u * RT;
-        v[u] = red_e[k < hi ? k : lo];
+        for (int u = 0; u < 2048 / RT; ++u) {
+          const int64_t k = lo + tg + (int64_t)u * RT;
+          v[u] = red_e[k < hi ? k : lo];
+        }
+#pragma unroll
+        for (int u = 0; u < 2048 / RT; ++u)
+          if (lo + tg + (int64_t)u * RT < hi) acc += v[u];
       }
-#pragma unroll
-      for (int u = 0; u < 2048 / RT; ++u)
-        if (lo + threadIdx.x + (int64_t)u * RT < hi) acc += v[u];
-    }
-    if (threadIdx.x < RT) {
       const double t = wave_sum(acc);
-      if ((threadIdx.x & 63) == 0) red_part[threadIdx.x >> 6] = t;
+      if ((tg & 63) == 0) red_part[g][tg >> 6] = t;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (sr < n_sys && tg == 0) {
       double t = 0.0;
-      for (int w = 0; w < RT / 64; ++w) t += red_part[w];
-      red_energy[s] = t;
-      red_gate[s] = 0;
-      if (s == 0) red_step[0] += 1;
+      for (int w = 0; w < RT / 64; ++w) t += red_part[g][w];
+      red_energy[sr] = t;
+      red_gate[sr] = 0;
+      if (sr == 0) red_step[0] += 1;
     }
+    return;
   }
   if (s >= n_sys || meta[0] > bin_cap || gated_off(gate, s)) return;
   const int32_t* gr = grid + (size_t)GRID * s;
@@ -1662,7 +1667,9 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
               "the build's per-system reduction needs energy, gate_clear and step");
   if (small) {
     // every system fits one workgroup: bin, scan, scatter and sort in one launch
-    k_build_small<<<n_systems, SB_THREADS, 0, st>>>(
+    // past the systems' workgroups, a quarter as many reduce energies
+    const int n_red = red ? (n_systems + SB_THREADS / 256 - 1) / (SB_THREADS / 256) : 0;
+    k_build_small<<<n_systems + n_red, SB_THREADS, 0, st>>>(
         pos, pos_stride, sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, w.bin_count,
         w.atom_bin, w.bin_start, order, w.wr, gate, xref, counter, cell_col, cell_build, jbits,
         w.cw, w.wmax, word_mode, frame ? frame->anchor : nullptr, frame ? frame->rel : nullptr,
