@@ -1038,8 +1038,10 @@ __device__ __forceinline__ bool half_keep(const WordCtx& c, uint32_t word, int i
 // Persistent grid of 256-thread workgroups walking the active-bin list, one
 // bin per workgroup at a time.  For one bin the workgroup enumerates the
 // stencil images (bin + lattice shift S, m_k bins each way, one image per
-// thread), scans their atom counts, and stages the candidates of all images
-// in LDS at once (FILL_CH at a time); a group of L lanes per centre atom
+// thread), scans their atom counts, writes each candidate's image into a
+// table, and stages the candidates of all images in LDS at once (FILL_CH at
+// a time; the next work item is loaded a bin ahead); a group of L lanes per
+// centre atom
 // (L * centre-atoms-per-round = 256, L <= 64) then tests candidates sub,
 // sub + L, ...; a ballot over the group orders the matches, so every row
 // lists its neighbours in candidate order -- image order (dx, dy, dz
