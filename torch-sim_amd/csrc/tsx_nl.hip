@@ -1041,9 +1041,9 @@ __device__ __forceinline__ bool half_keep(const WordCtx& c, uint32_t word, int i
 // thread), scans their atom counts, writes each candidate's image into a
 // table, and stages the candidates of all images in LDS at once (FILL_CH at
 // a time; the next work item is loaded a bin ahead); a group of L lanes per
-// centre atom
-// (L * centre-atoms-per-round = 256, L <= 64) then tests candidates sub,
-// sub + L, ...; a ballot over the group orders the matches, so every row
+// centre atom (L * centre-atoms-per-round = 256, L <= 64) then tests
+// candidates sub, sub + L, ...; a ballot over the group orders the matches,
+// so every row
 // lists its neighbours in candidate order -- image order (dx, dy, dz
 // nested), then atom index -- for any L (deterministic).
 constexpr int FILL_T = 256;    // threads per block
