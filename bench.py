@@ -42,6 +42,8 @@ MORSE = dict(sigma=2.866, epsilon=0.3429, alpha=1.3588)
 MEASURED_TRAFFIC = {
     ("cu_morse_nvt", 0.6, 2): {"fetch_kib": 111967.1, "fetch_factor": 2.000,
                                "write_kib": 41957.1, "write_factor": 1.000},
+    ("cu_morse_nvt", 0.66, 2): {"fetch_kib": 117702.1, "fetch_factor": 2.000,
+                                "write_kib": 41769.9, "write_factor": 1.000},
     ("cu_morse_nvt", 0.72, 2): {"fetch_kib": 123122.7, "fetch_factor": 2.000,
                                 "write_kib": 41832.4, "write_factor": 1.000},
     ("lj1m_nve", 0.3, 4): {"fetch_kib": 342936.1, "fetch_factor": 2.000,

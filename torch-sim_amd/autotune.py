@@ -77,7 +77,7 @@ def tune_md(
         return disk[key]
     rc = model.cutoff
     if skins is None:
-        skins = tuple(round(f * rc, 3) for f in (0.05, 0.08, 0.10, 0.12, 0.16))
+        skins = tuple(round(f * rc, 3) for f in (0.05, 0.08, 0.10, 0.11, 0.12, 0.16))
     kT = kT_kelvin * MetalUnits.temperature
     dt = 0.002 * MetalUnits.time
     st = state.clone()
