@@ -169,6 +169,29 @@ def test_rebuild_launch_reduction_equals_own_launch(dev):
         assert torch.equal(a.energy, b.energy)
 
 
+def test_autotuner_picks_from_its_grid_and_caches(dev, tmp_path, monkeypatch):
+    """tune_md times every (tpa, skin) pair of its grid, re-times the two
+    fastest and returns one of them; the choice is cached per workload shape,
+    in memory and in the JSON file, so a later call runs no trial."""
+    import json
+    from torch_sim_amd import autotune
+    monkeypatch.setenv("TSX_AUTOTUNE_CACHE", str(tmp_path / "at.json"))
+    monkeypatch.setattr(autotune, "_MEM", {})
+    st, model, md, kT = _setup(dev, n_sys=4, seed=41)
+    lines = []
+    cfg = autotune.tune_md(st, model, tpas=(2, 4), skins=(0.3, 0.6), warmup_steps=5,
+                           trial_steps=10, log=lines.append)
+    print(lines, cfg, flush=True)
+    assert (cfg["skin"], cfg["tpa"]) in {(0.3, 2), (0.3, 4), (0.6, 2), (0.6, 4)}
+    assert len(lines) == 6 and sum("re-timed" in ln for ln in lines) == 2
+    assert cfg["rate"] > 0
+    with open(tmp_path / "at.json") as fh:
+        assert list(json.load(fh).values()) == [cfg]
+    autotune._MEM.clear()
+    again = []
+    assert autotune.tune_md(st, model, log=again.append) == cfg and not again
+
+
 def test_engine_graph_replay_equals_eager(dev):
     st, model, md, kT = _setup(dev, seed=24)
     dt = 0.002 * tsa.MetalUnits.time
