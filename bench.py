@@ -567,6 +567,20 @@ def bench_fit(args) -> dict:
                   "force_matching_only_grad_evals_per_s": fm})
 
 
+def calibrate_copy(mib: int) -> None:
+    """Three device-to-device copies of ``mib`` MiB: each reads and writes
+    exactly that many bytes, so a byte counter read under rocprofv3 --pmc
+    divided by the expected KiB is the factor applied to the counters of the
+    measured kernels."""
+    n = mib * 1024 * 1024 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"copied {mib} MiB x3", flush=True)
+
+
 WORKLOADS = {
     "cu_morse_nvt": (bench_cu_morse, 1000, 50),
     "lj1m_nve": (bench_lj1m, 500, 20),
@@ -590,7 +604,13 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-systems", type=int, default=32)
     ap.add_argument("--fit-md-steps", type=int, default=200)
+    ap.add_argument("--calibrate-copy", type=int, default=None, metavar="MIB",
+                    help="only run three device copies of MIB MiB (a kernel of known bytes "
+                         "for scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy is not None:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU; runs on more than one GPU are not part of this build "
               "(independent systems shard across GPUs without communication).", file=sys.stderr)

@@ -42,8 +42,19 @@ PINNING = {
     "test_soft_sphere.py; finite differences",
     "models.pair_forward": "perfect-lattice closed sums (E/atom, F=0); F=-dE/dx and "
     "stress=-(1/V)dE/deps by finite differences; half==full; sum F=0",
+    "models.MultiPairFn": "species lookup restated from test_soft_sphere.py:84-116 with exact "
+    "values (0.125 / 0.28125 / 0.3472); row order follows the atomic_numbers list; eps and "
+    "alpha matrices used; LJ branch V_ab(2^(1/6) sigma_ab) = -eps_ab with zero slope; dimers "
+    "of every species pair through pair_forward against the written-out LJ energy and force",
+    "models.pair_forward per-atom energies / stresses": "perfect fcc lattice: every atom's "
+    "energy is the full shell sum, the stress is N/(6V) sum_s n_s r_s V'(r_s) I and each "
+    "atom carries 1/N of it (at a = 3.61 and compressed 3.50); on a jittered triclinic batch "
+    "sum energies = 2E and sum stresses = the system stress per system",
     "md.*": "BAOAB one-step closed form with supplied noise; NVE energy drift; "
     "COM removal; batch==single",
+    "md.kinetic_energy / md.calc_kT": "constructed momenta p = sqrt(m) u give KE = 0.5 sum "
+    "|u|^2 and kT = sum |u|^2 / (3n) per system; Maxwell-Boltzmann momenta at kT with the COM "
+    "removed measure kT (n-1)/n",
     "fire.ase_fire_step": "P<0 branch (test_optimizers.py:600-662); mixing known "
     "answers with per-system norms; parallel-velocity identity; dt_max and max_step "
     "clamps; first step uses raw forces; Frechet first step = scalar exp scaling",

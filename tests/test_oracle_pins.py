@@ -1,0 +1,208 @@
+"""Closed-form pins of the oracle pieces that the GPU tests lean on and that
+no reference golden vector covers directly: the species-pair lookup, the
+per-atom energies and stresses, and the kinetic-energy / temperature
+reductions (CPU only).
+
+Every expected value here is computed from the mathematics of the case (a
+pair formula evaluated by hand, an fcc shell sum, a constructed momentum
+set), never by calling the oracle function under test, so that a plausible
+mistake in the oracle -- a transposed species index, a dropped factor 0.5, a
+wrong sign, a per-atom weight of 1 instead of 1/2 -- fails one of them.
+"""
+
+import math
+
+import pytest
+import torch
+
+from oracle import md as omd, models as om
+
+F64 = torch.float64
+
+
+# ----------------------------------------------------- species-pair lookup
+def _mss(sigma, epsilon=None, alpha=None, kind="soft_sphere", z=(18, 36)):
+    n = len(z)
+    eps = torch.ones(n, n, dtype=F64) if epsilon is None else torch.tensor(epsilon, dtype=F64)
+    alp = None if alpha is None else torch.tensor(alpha, dtype=F64)
+    return om.MultiPairFn(list(z), torch.tensor(sigma, dtype=F64), eps, alp, kind=kind)
+
+
+def test_multi_soft_sphere_species_lookup_exact_values():
+    """tests/models/test_soft_sphere.py:84-104 with the values written out:
+    V = eps/alpha (1 - r/sigma)^alpha at r = 0.5 with sigma_AA = 1,
+    sigma_AB = 2, sigma_BB = 3, eps = 1, alpha = 2."""
+    fn = _mss([[1.0, 2.0], [2.0, 3.0]], alpha=[[2.0, 2.0], [2.0, 2.0]])
+    dr = torch.full((4,), 0.5, dtype=F64)
+    zi = torch.tensor([18, 18, 36, 36])
+    zj = torch.tensor([18, 36, 18, 36])
+    e = fn(dr, zi, zj)
+    assert e.tolist() == [0.125, 0.28125, 0.28125, 0.5 * (1.0 - 0.5 / 3.0) ** 2]
+
+
+def test_multi_soft_sphere_species_order_follows_atomic_numbers_list():
+    """Rows follow the order of ``atomic_numbers`` (soft_sphere.py:132-140):
+    listing Kr first makes sigma[0, 0] the Kr-Kr value."""
+    fn = _mss([[3.0, 2.0], [2.0, 1.0]], z=(36, 18), alpha=[[2.0, 2.0], [2.0, 2.0]])
+    dr = torch.full((2,), 0.5, dtype=F64)
+    e = fn(dr, torch.tensor([18, 36]), torch.tensor([18, 36]))
+    assert e.tolist() == [0.125, 0.5 * (1.0 - 0.5 / 3.0) ** 2]
+
+
+def test_multi_soft_sphere_alpha_and_epsilon_matrices_are_used():
+    """eps_AB = 3, alpha_AB = 3 at r = 0.5, sigma_AB = 1: 3/3 (1/2)^3 = 0.125;
+    eps_AA = 2, alpha_AA = 4: 2/4 (1/2)^4 = 0.03125."""
+    fn = _mss([[1.0, 1.0], [1.0, 1.0]], epsilon=[[2.0, 3.0], [3.0, 1.0]],
+              alpha=[[4.0, 3.0], [3.0, 2.0]])
+    dr = torch.full((2,), 0.5, dtype=F64)
+    e = fn(dr, torch.tensor([18, 18]), torch.tensor([36, 18]))
+    assert e.tolist() == [0.125, 0.03125]
+
+
+def test_multi_lennard_jones_minimum_is_minus_epsilon_of_the_pair():
+    """V_ab(2^(1/6) sigma_ab) = -eps_ab for each species pair, and zero force
+    there (the LJ branch of the lookup, Kob-Andersen parameters)."""
+    sig = [[1.0, 0.8], [0.8, 0.88]]
+    eps = [[1.0, 1.5], [1.5, 0.5]]
+    fn = _mss(sig, epsilon=eps, kind="lennard_jones", z=(1, 2))
+    for a, b in [(1, 1), (1, 2), (2, 1), (2, 2)]:
+        s, e = sig[a - 1][b - 1], eps[a - 1][b - 1]
+        r = torch.tensor([2.0 ** (1.0 / 6.0) * s], dtype=F64, requires_grad=True)
+        v = fn(r, torch.tensor([a]), torch.tensor([b]))
+        assert abs(v.item() + e) < 1e-14
+        (g,) = torch.autograd.grad(v.sum(), r)
+        assert abs(g.item()) < 1e-12
+
+
+def test_multi_species_dimers_energy_and_force_closed_form():
+    """pair_forward on isolated dimers of each species pair (one system per
+    dimer, non-periodic): E = V_ab(r) and F_j = -V_ab'(r) r_hat, written out
+    from the LJ formula with the pair's own (sigma, epsilon)."""
+    sig = [[1.0, 0.8], [0.8, 0.88]]
+    eps = [[1.0, 1.5], [1.5, 0.5]]
+    fn = _mss(sig, epsilon=eps, kind="lennard_jones", z=(7, 9))
+    pairs = [(7, 7, 1.05), (7, 9, 0.93), (9, 7, 1.21), (9, 9, 0.97)]
+    pos, zs, cells = [], [], []
+    u = torch.tensor([0.6, 0.0, 0.8], dtype=F64)
+    for k, (za, zb, r) in enumerate(pairs):
+        base = torch.tensor([2.0, 3.0, 1.0], dtype=F64)
+        pos += [base, base + r * u]
+        zs += [za, zb]
+        cells.append(torch.eye(3, dtype=F64) * 20.0)
+    pos = torch.stack(pos)
+    z = torch.tensor(zs)
+    idx = torch.arange(len(pairs)).repeat_interleave(2)
+    out = om.pair_forward(pos, torch.stack(cells), torch.tensor([False] * 3), z, idx, fn, 2.5,
+                          per_atom=True)
+    for k, (za, zb, r) in enumerate(pairs):
+        s, e = sig[za // 9][zb // 9], eps[za // 9][zb // 9]
+        v = 4 * e * ((s / r) ** 12 - (s / r) ** 6)
+        dv = 4 * e * (-12 * s**12 / r**13 + 6 * s**6 / r**7)
+        assert abs(out["energy"][k].item() - v) < 1e-13
+        fj = -dv * u
+        torch.testing.assert_close(out["forces"][2 * k + 1], fj, rtol=1e-12, atol=1e-13)
+        torch.testing.assert_close(out["forces"][2 * k], -fj, rtol=1e-12, atol=1e-13)
+        # each atom of the pair carries the full pair energy (reference quirk)
+        assert abs(out["energies"][2 * k].item() - v) < 1e-13
+        assert abs(out["energies"][2 * k + 1].item() - v) < 1e-13
+
+
+# --------------------------------------------------- per-atom energy / stress
+SHELLS = [(1, 12), (2, 6), (3, 24), (4, 12), (5, 24), (6, 8), (7, 48), (8, 6)]
+MORSE = dict(sigma=2.866, epsilon=0.3429, alpha=1.3588)
+
+
+def _fcc(a, reps):
+    basis = torch.tensor([[0, 0, 0], [0, 0.5, 0.5], [0.5, 0, 0.5], [0.5, 0.5, 0]], dtype=F64)
+    g = torch.stack(torch.meshgrid(*[torch.arange(r, dtype=F64) for r in reps], indexing="ij"),
+                    -1).reshape(-1, 1, 3)
+    return ((g + basis[None]).reshape(-1, 3)) * a
+
+
+def _morse_v(r):
+    s, e, a = MORSE["sigma"], MORSE["epsilon"], MORSE["alpha"]
+    return e * (1 - math.exp(-a * (r - s))) ** 2 - e
+
+
+def _morse_dv(r):
+    s, e, a = MORSE["sigma"], MORSE["epsilon"], MORSE["alpha"]
+    x = math.exp(-a * (r - s))
+    return 2 * e * a * x * (1 - x)
+
+
+@pytest.mark.parametrize("a", [3.61, 3.50])
+def test_perfect_lattice_per_atom_energy_and_stress_closed_form(a):
+    """Cu fcc 3x3x3 (108 atoms, rc = 6): every atom's energy is the full shell
+    sum sum_s n_s V(r_s) (per-atom energies count each pair at both ends);
+    the system stress is isotropic, N / (6 V) sum_s n_s r_s V'(r_s), and
+    each atom carries 1/N of it.  a = 3.50 compresses the crystal so that the
+    pressure term is large (a sign error would flip it)."""
+    rc = 6.0
+    pos = _fcc(a, (3, 3, 3))
+    n = pos.shape[0]
+    cell = (torch.eye(3, dtype=F64) * 3 * a).unsqueeze(0)
+    fn = lambda d, zi, zj: om.morse_pair(d, zi, zj, **MORSE)  # noqa: E731
+    out = om.pair_forward(pos, cell, torch.tensor([True] * 3), torch.ones(n, dtype=torch.long),
+                          torch.zeros(n, dtype=torch.long), fn, rc, per_atom=True)
+    shells = [(a * math.sqrt(k / 2), c) for k, c in SHELLS if a * math.sqrt(k / 2) < rc]
+    e_atom = sum(c * _morse_v(r) for r, c in shells)
+    assert (out["energies"] - e_atom).abs().max() < 1e-12
+    assert abs(out["energies"].sum().item() - 2 * out["energy"].item()) < 1e-10
+    vol = (3 * a) ** 3
+    p = n / (6 * vol) * sum(c * r * _morse_dv(r) for r, c in shells)
+    torch.testing.assert_close(out["stress"][0], p * torch.eye(3, dtype=F64), rtol=1e-11, atol=1e-15)
+    expect_atom = (p / n) * torch.eye(3, dtype=F64)
+    assert (out["stresses"] - expect_atom).abs().max() < 1e-14 + 1e-11 * abs(p / n)
+
+
+def test_per_atom_outputs_sum_to_system_values_on_a_jittered_batch():
+    """Two jittered triclinic systems: sum of per-atom energies = 2 E and
+    sum of per-atom stresses = the system stress, per system (the weights
+    1 and 1/2 of pair_potential.py:411-417 and 253-264)."""
+    g = torch.Generator().manual_seed(4)
+    pos1 = _fcc(3.61, (2, 2, 3)) + 0.08 * torch.randn(48, 3, generator=g, dtype=F64)
+    pos2 = _fcc(3.61, (3, 2, 2)) + 0.08 * torch.randn(48, 3, generator=g, dtype=F64)
+    c1 = torch.tensor([[7.22, 0.3, 0.0], [0.0, 7.22, 0.2], [0.1, 0.0, 10.83]], dtype=F64)
+    c2 = torch.tensor([[10.83, 0.0, -0.2], [0.4, 7.22, 0.0], [0.0, 0.0, 7.22]], dtype=F64)
+    pos = torch.cat([pos1, pos2])
+    idx = torch.arange(2).repeat_interleave(48)
+    fn = lambda d, zi, zj: om.morse_pair(d, zi, zj, **MORSE)  # noqa: E731
+    out = om.pair_forward(pos, torch.stack([c1, c2]), torch.tensor([True] * 3),
+                          torch.ones(96, dtype=torch.long), idx, fn, 6.0, per_atom=True)
+    for s in range(2):
+        sel = idx == s
+        assert abs(out["energies"][sel].sum().item() - 2 * out["energy"][s].item()) < 1e-11
+        torch.testing.assert_close(out["stresses"][sel].sum(0), out["stress"][s], rtol=1e-11,
+                                   atol=1e-14)
+
+
+# ------------------------------------------------------------ calc_kT / KE
+def test_kinetic_energy_and_kT_closed_form():
+    """Momenta p_i = sqrt(m_i) u_i with chosen u: per-system
+    KE = 0.5 sum |u_i|^2 and kT = sum |u_i|^2 / (3 n) (quantities.py:16-66,
+    104-139, dof = 3 n)."""
+    u = torch.tensor([[1.0, 0.0, 0.0], [0.0, 2.0, 0.0], [0.0, 0.0, 3.0],
+                      [1.0, 1.0, 1.0], [2.0, -1.0, 2.0]], dtype=F64)
+    m = torch.tensor([1.0, 4.0, 9.0, 2.0, 0.5], dtype=F64)
+    idx = torch.tensor([0, 0, 0, 1, 1])
+    p = torch.sqrt(m).unsqueeze(-1) * u
+    ke = omd.kinetic_energy(p, m, idx, 2)
+    torch.testing.assert_close(ke, torch.tensor([0.5 * 14.0, 0.5 * 12.0], dtype=F64),
+                               rtol=1e-15, atol=0)
+    kt = omd.calc_kT(p, m, idx, 2)
+    torch.testing.assert_close(kt, torch.tensor([14.0 / 9.0, 12.0 / 6.0], dtype=F64),
+                               rtol=1e-15, atol=0)
+
+
+def test_initialized_momenta_have_the_requested_temperature():
+    """Maxwell-Boltzmann momenta at kT with the centre-of-mass momentum
+    removed: E[kT_measured] = kT (n - 1) / n, checked on 20,000 atoms of
+    unequal mass at 0.3 % (about 10 standard errors)."""
+    g = torch.Generator().manual_seed(7)
+    n = 20000
+    m = 1.0 + 60.0 * torch.rand(n, generator=g, dtype=F64)
+    idx = torch.zeros(n, dtype=torch.long)
+    kT = torch.tensor(0.0862, dtype=F64)
+    p = omd.initialize_momenta(m, idx, kT, torch.randn(n, 3, generator=g, dtype=F64))
+    kt = omd.calc_kT(p, m, idx, 1).item()
+    assert abs(kt / (0.0862 * (n - 1) / n) - 1.0) < 3e-3

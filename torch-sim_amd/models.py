@@ -24,7 +24,7 @@ import torch
 
 from . import _lib
 from ._lib import call, ptr, stream
-from .neighbors import VerletList, NeighborList, SystemIndex, torchsim_nl
+from .neighbors import NeighborList, SystemIndex, TensorKey, VerletList, torchsim_nl
 
 LJ, MORSE, SOFT_SPHERE = 0, 1, 2
 
@@ -197,15 +197,14 @@ class PairPotentialModel(ModelInterface):
         their parameter table with it)."""
         if self.n_species == 1:
             return None
-        key = (atomic_numbers.data_ptr(), atomic_numbers._version, atomic_numbers.shape[0])
-        if key == self._species_key:
+        if self._species_key is not None and self._species_key.matches(atomic_numbers):
             return self._species
         if int(atomic_numbers.max()) >= self._z_to_idx.numel() or int(atomic_numbers.min()) < 0:
             raise ValueError("atomic number outside the model's species table")
         sp = self._z_to_idx[atomic_numbers].contiguous()
         if int(sp.min()) < 0:
             raise ValueError("state contains an atomic number that is not a model species")
-        self._species_key, self._species = key, sp
+        self._species_key, self._species = TensorKey(atomic_numbers), sp
         return sp
 
     def forward(self, state, **_kwargs) -> dict[str, torch.Tensor]:

@@ -20,7 +20,9 @@ reused by the models and the batched engine.
 from __future__ import annotations
 
 import ctypes
+import itertools
 import math
+import weakref
 
 import numpy as np
 import torch
@@ -35,9 +37,36 @@ STATUS_SINGULAR_CELL = 8
 STATUS_BAD_SIZE = 32
 
 
+class TensorKey:
+    """Identity of a tensor's contents for cache lookups.
+
+    It holds the tensor object through a weak reference plus its in-place
+    version counter and shape.  A freed tensor never matches, even when the
+    caching allocator hands its memory to a new tensor of the same size (a
+    key made of the data pointer would match that new tensor and return stale
+    results); an in-place write bumps the version and misses too.
+    """
+
+    __slots__ = ("_ref", "_version", "_shape")
+
+    def __init__(self, t: torch.Tensor):
+        self._ref = weakref.ref(t)
+        self._version = t._version
+        self._shape = tuple(t.shape)
+
+    def matches(self, t: torch.Tensor) -> bool:
+        return self._ref() is t and t._version == self._version and tuple(t.shape) == self._shape
+
+
+_SERIAL = itertools.count(1)
+
+
 class SystemIndex:
     """Device bookkeeping of the system layout: int32 atom->system map and the
-    [S+1] atom offsets of each system (no host synchronisation)."""
+    [S+1] atom offsets of each system (no host synchronisation).
+
+    ``serial`` is unique per instance: lists built for one layout record it
+    and rebuild in full when they are handed another."""
 
     def __init__(self, system_idx: torch.Tensor, n_systems: int):
         dev = system_idx.device
@@ -46,7 +75,8 @@ class SystemIndex:
         self.atom_sys = system_idx.to(torch.int32).contiguous()
         bounds = torch.arange(self.n_systems + 1, device=dev, dtype=system_idx.dtype)
         self.sys_start = torch.searchsorted(system_idx, bounds).to(torch.int32).contiguous()
-        self._key = (system_idx.data_ptr(), system_idx._version, self.n_atoms, self.n_systems)
+        self._source = TensorKey(system_idx)
+        self.serial = next(_SERIAL)
         self._counts = None
         self._red_ws = None
 
@@ -67,12 +97,9 @@ class SystemIndex:
         return self._red_ws
 
     def matches(self, system_idx: torch.Tensor, n_systems: int) -> bool:
-        return self._key == (
-            system_idx.data_ptr(),
-            system_idx._version,
-            int(system_idx.shape[0]),
-            int(n_systems),
-        )
+        """True when ``system_idx`` is the very tensor this index was made
+        from, unmodified since."""
+        return self._source.matches(system_idx) and int(n_systems) == self.n_systems
 
     @property
     def counts(self) -> np.ndarray:
@@ -348,7 +375,7 @@ class VerletList:
         self.ws = torch.empty((nbytes + 7) // 8, dtype=torch.float64, device=dev)
         self.nl.build(pos, cell_col, pbc, sysidx, pos_stride, xref=self.xref,
                       cell_build=self.cell_build)
-        self._key = (sysidx._key, sysidx.n_atoms, sysidx.n_systems)
+        self._key = sysidx.serial
         self.full_builds += 1
         self._gen += 1
 
@@ -358,8 +385,7 @@ class VerletList:
         return (self._gen, self.nl.generation)
 
     def ready(self, sysidx: SystemIndex) -> bool:
-        return self.xref is not None and self._key == (sysidx._key, sysidx.n_atoms,
-                                                        sysidx.n_systems)
+        return self.xref is not None and self._key == sysidx.serial
 
     def rebuild_all(self, pos: torch.Tensor, cell_col: torch.Tensor, pbc: torch.Tensor,
                     sysidx: SystemIndex, pos_stride: int = 3) -> NeighborList:
@@ -414,8 +440,7 @@ class VerletList:
         """
         self.updates += 1
         self._pending = None
-        key = (sysidx._key, sysidx.n_atoms, sysidx.n_systems)
-        if self._key != key or self.xref is None:
+        if self._key != sysidx.serial or self.xref is None:
             self._full(pos, cell_col, pbc, sysidx, pos_stride)
             return self.nl
         if sysidx.n_atoms == 0:
