@@ -182,6 +182,33 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
                  int32_t* counter, double* cell_build, int32_t fill_mode, int32_t word_mode,
                  const tsx_nl_frame* frame, int32_t max_system_atoms, void* stream);
 
+/* tsx_nl_build without a workspace argument: no counterpart in the
+ * reference.  It is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the workspace is a static device array of 36 MiB inside the
+ * library, one per GPU, held until the process ends; the call itself
+ * allocates nothing.  Limits: n_atoms <= TSX_NOALLOC_MAX_ATOMS, n_systems <=
+ * TSX_NOALLOC_MAX_SYSTEMS, bin_cap <= TSX_NOALLOC_MAX_BINS; larger sizes
+ * return TSX_ERR_ARG with a message.  Every call rebuilds the per-system
+ * geometry (setup = 1 of tsx_nl_build), because the array keeps whatever the
+ * previous call on the device left in it; outputs are those of tsx_nl_build
+ * with setup = 1.  All calls on one GPU share the array, so calls that
+ * overlap on that GPU (two streams, two threads) race: serialise them.  The
+ * first call on each device looks the array's address up
+ * (hipGetSymbolAddress); make that first call outside graph capture. */
+#define TSX_NOALLOC_MAX_ATOMS 524288
+#define TSX_NOALLOC_MAX_SYSTEMS 4096
+#define TSX_NOALLOC_MAX_BINS 524288
+int tsx_nl_build_noalloc(const double* pos, int32_t pos_stride, const double* cell_col,
+                         const uint8_t* pbc, int32_t pbc_per_system, const int32_t* sys_start,
+                         const int32_t* atom_sys, int64_t n_atoms, int32_t n_systems,
+                         double r_list, int32_t bins_per_rlist, int32_t self_interaction,
+                         int32_t half, int32_t jbits, int32_t tpa, int32_t kmax, int64_t ld,
+                         int32_t bin_cap, uint32_t* nbr, int32_t* count, int32_t* order,
+                         double* xref, int32_t* status, const int32_t* gate, int32_t* counter,
+                         double* cell_build, int32_t fill_mode, int32_t word_mode,
+                         const tsx_nl_frame* frame, int32_t max_system_atoms, void* stream);
+
 /* Exclusive scan of ``count`` into ``row_ptr`` [n+1] (int64); the total is
  * row_ptr[n].  Used before tsx_nl_emit_pairs.  ``ws`` holds
  * tsx_scan_ws_bytes(n) bytes. */

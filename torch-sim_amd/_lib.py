@@ -36,6 +36,11 @@ _SIGS = {
         [P, I, P, P, I, P, P, L, I, D, I, I, I, I, I, I, L, I, I, P, P, P, P, P, P, P, P, P, I, I,
          P, I, P],
     ),
+    "tsx_nl_build_noalloc": (
+        I,
+        [P, I, P, P, I, P, P, L, I, D, I, I, I, I, I, I, L, I, P, P, P, P, P, P, P, P, I, I, P, I,
+         P],
+    ),
     "tsx_nl_check_ws_bytes": (L, [I]),
     "tsx_nl_check": (I, [P, I, P, P, P, P, L, I, D, D, P, P, P]),
     "tsx_scan_ws_bytes": (L, [L]),

@@ -23,6 +23,7 @@
 #include "tsx_mat33.h"
 
 #include <cstdarg>
+#include <mutex>
 
 namespace tsx {
 
@@ -1580,6 +1581,43 @@ __global__ void k_check_atoms(const double* __restrict__ pos, int ps,
   if (!(dx * dx + dy * dy + dz * dz <= w[9])) flags[s] = 1;
 }
 
+// ------------------------------------------------- no-allocation build
+// Workspace of tsx_nl_build_noalloc: a file-scope device array, one copy per
+// GPU, held until the process ends.  In host code ``scratch`` names a host
+// stand-in, not device memory: its device address comes from
+// hipGetSymbolAddress, looked up once per device (with that device current)
+// and kept in g_scratch_ptr; only that address is handed to kernels.
+constexpr size_t NOALLOC_WS_BYTES = 36ull << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_WS_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of ``scratch`` on the current device (nullptr and an error
+// message on failure).
+static void* scratch_on_current_device() {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+    set_error("tsx_nl_build_noalloc: no usable current device (%s, device %d)",
+              hipGetErrorString(e), dev);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      set_error("tsx_nl_build_noalloc: hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+
 }  // namespace tsx
 
 using namespace tsx;
@@ -1608,6 +1646,35 @@ int tsx_nl_check(const double* pos, int32_t pos_stride, const double* cell_col,
     TSX_CHECK_LAUNCH();
   }
   return TSX_OK;
+}
+
+int tsx_nl_build_noalloc(const double* pos, int32_t pos_stride, const double* cell_col,
+                         const uint8_t* pbc, int32_t pbc_per_system, const int32_t* sys_start,
+                         const int32_t* atom_sys, int64_t n_atoms, int32_t n_systems,
+                         double r_list, int32_t bins_per_rlist, int32_t self_interaction,
+                         int32_t half, int32_t jbits, int32_t tpa, int32_t kmax, int64_t ld,
+                         int32_t bin_cap, uint32_t* nbr, int32_t* count, int32_t* order,
+                         double* xref, int32_t* status, const int32_t* gate, int32_t* counter,
+                         double* cell_build, int32_t fill_mode, int32_t word_mode,
+                         const tsx_nl_frame* frame, int32_t max_system_atoms, void* stream) {
+  TSX_REQUIRE(n_atoms >= 0 && n_atoms <= TSX_NOALLOC_MAX_ATOMS,
+              "tsx_nl_build_noalloc: n_atoms %lld outside [0, %d]", (long long)n_atoms,
+              TSX_NOALLOC_MAX_ATOMS);
+  TSX_REQUIRE(n_systems > 0 && n_systems <= TSX_NOALLOC_MAX_SYSTEMS,
+              "tsx_nl_build_noalloc: n_systems %d outside [1, %d]", n_systems,
+              TSX_NOALLOC_MAX_SYSTEMS);
+  TSX_REQUIRE(bin_cap >= 1 && bin_cap <= TSX_NOALLOC_MAX_BINS,
+              "tsx_nl_build_noalloc: bin_cap %d outside [1, %d]", bin_cap, TSX_NOALLOC_MAX_BINS);
+  TSX_REQUIRE(ws_layout(n_atoms, n_systems, bin_cap, nullptr, nullptr) <= NOALLOC_WS_BYTES,
+              "tsx_nl_build_noalloc: workspace exceeds the static scratch");
+  void* ws = scratch_on_current_device();
+  if (!ws) return TSX_ERR_HIP;
+  // the scratch holds whatever the last call on this device left: rebuild the
+  // per-system geometry and clear the bin counters every time (setup = 1)
+  return tsx_nl_build(pos, pos_stride, cell_col, pbc, pbc_per_system, sys_start, atom_sys,
+                      n_atoms, n_systems, r_list, bins_per_rlist, self_interaction, half, jbits,
+                      tpa, kmax, ld, bin_cap, 1, ws, nbr, count, order, xref, status, gate,
+                      counter, cell_build, fill_mode, word_mode, frame, max_system_atoms, stream);
 }
 
 int tsx_version(void) { return 100; }
