@@ -26,12 +26,17 @@ from .models import (
     ModelInterface,
     MorseModel,
     MultiLennardJonesModel,
+    MultiLennardJonesPairFn,
+    MultiSoftSpherePairFn,
     PairPotentialModel,
     SoftSphereModel,
     SoftSphereMultiModel,
+    lennard_jones_pair,
+    morse_pair,
+    soft_sphere_pair,
 )
-from .neighbors import torchsim_nl
-from .quantities import calc_kinetic_energy, calc_kT, system_wise_max_force
+from .neighbors import default_batched_nl, torch_nl_linked_cell, torch_nl_n2, torchsim_nl
+from .quantities import calc_kinetic_energy, calc_kT, calc_temperature, system_wise_max_force
 from .optimizers import (
     OPTIM_REGISTRY,
     CellFilter,

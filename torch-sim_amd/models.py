@@ -2,16 +2,32 @@
 
 ``forward(state) -> {"energy" [S], "forces" [N,3], "stress" [S,3,3],
 "energies" [N], "stresses" [N,3,3]}`` as torch_sim/models/interface.py:169-201
-and pair_potential.py:351-464, with the constructor keywords of
-LennardJonesModel (lennard_jones.py:92-106), MorseModel (morse.py:96-111),
-SoftSphereModel (soft_sphere.py:192-208) and SoftSphereMultiModel
-(soft_sphere.py:265-283).
+and pair_potential.py:351-464.  ``PairPotentialModel(pair_fn, *, cutoff,
+...)`` takes the reference's constructor (pair_potential.py:300-345); the
+convenience models build their pair function with ``functools.partial``
+exactly as LennardJonesModel (lennard_jones.py:92-137), MorseModel
+(morse.py:96-144), SoftSphereModel (soft_sphere.py:192-246) and
+SoftSphereMultiModel (soft_sphere.py:265-377) do.
+
+The pair function is not called during a forward pass: the model recognises
+the closed forms (``lennard_jones_pair``, ``morse_pair``,
+``soft_sphere_pair`` bound with ``functools.partial``, and the species-pair
+``MultiSoftSpherePairFn`` / ``MultiLennardJonesPairFn``) and hands their
+parameters to the HIP pair kernel (FP32 pair arithmetic, FP64
+accumulation).  Any other callable is refused with a TypeError.
 
 Each call rebuilds the neighbor list (as the reference does,
-pair_potential.py:168-174) with the HIP cell list and evaluates the pairs
-with the HIP pair kernel (FP32 pair arithmetic, FP64 accumulation).  The
-kernel walks full lists (each pair from both ends, no atomics); the result
-equals the reference's half-list sum.
+pair_potential.py:168-174) with the HIP cell list, or keeps a Verlet list
+with ``skin``.  The kernel walks full lists (each pair from both ends, no
+atomics); the result equals the reference's half-list sum, so
+``reduce_to_half_list`` changes nothing but is accepted.
+
+``retain_graph=True`` (pair_potential.py:419-425, 461-462): ``energy`` and
+``forces`` come back attached to autograd through the HIP backward kernels
+(positions, cell and the pair function's parameter tensors); the forces'
+graph includes the second-derivative term that the reference's
+``create_graph=False`` leaves out.  Per-atom outputs and stress stay
+detached.
 
 Reference quirk kept: per-atom ``energies`` give every atom the full energy
 of each of its pairs (pair_potential.py:411-417), so they sum to twice the
@@ -20,13 +36,142 @@ system energy.
 
 from __future__ import annotations
 
+import functools
+import inspect
+
 import torch
 
 from . import _lib
 from ._lib import call, ptr, stream
-from .neighbors import NeighborList, SystemIndex, TensorKey, VerletList, torchsim_nl
+from .neighbors import (
+    NeighborList,
+    SystemIndex,
+    TensorKey,
+    VerletList,
+    default_batched_nl,
+    torch_nl_linked_cell,
+    torch_nl_n2,
+    torchsim_nl,
+)
 
 LJ, MORSE, SOFT_SPHERE = 0, 1, 2
+
+# every name the reference gives its batched neighbor list; on this backend
+# they are all the HIP cell-list search
+_NL_FNS = (torchsim_nl, torch_nl_linked_cell, torch_nl_n2, default_batched_nl)
+
+
+# ------------------------------------------------------------ pair functions
+def lennard_jones_pair(dr, zi, zj, sigma=1.0, epsilon=1.0):  # noqa: ARG001
+    """V = 4 eps [(s/r)^12 - (s/r)^6], 0 at r <= 0 (lennard_jones.py:23-47)."""
+    idr6 = (sigma / dr).pow(6)
+    energy = 4.0 * epsilon * (idr6 * idr6 - idr6)
+    return torch.where(dr > 0, energy, torch.zeros_like(energy))
+
+
+def morse_pair(dr, zi, zj, sigma=1.0, epsilon=5.0, alpha=5.0):  # noqa: ARG001
+    """V = eps (1 - exp(-a (r - s)))^2 - eps, 0 at r <= 0 (morse.py:23-47)."""
+    energy = epsilon * (1.0 - torch.exp(-alpha * (dr - sigma))).pow(2) - epsilon
+    return torch.where(dr > 0, energy, torch.zeros_like(energy))
+
+
+def soft_sphere_pair(dr, zi, zj, sigma=1.0, epsilon=1.0, alpha=2.0):  # noqa: ARG001
+    """V = eps/a (1 - r/s)^a for r < s, else 0 (soft_sphere.py:35-59)."""
+    energy = epsilon / alpha * (1.0 - dr / sigma).pow(alpha)
+    return torch.where(dr < sigma, energy, torch.zeros_like(energy))
+
+
+class _SpeciesPairFn(torch.nn.Module):
+    """Per-species-pair parameter matrices indexed through the atomic numbers
+    (row k belongs to ``atomic_numbers[k]``, soft_sphere.py:132-140)."""
+
+    _names = ("sigma_matrix", "epsilon_matrix", "alpha_matrix")
+
+    def __init__(self, atomic_numbers, sigma_matrix, epsilon_matrix, alpha_matrix=None,
+                 alpha_default=2.0):
+        super().__init__()
+        atomic_numbers = torch.as_tensor(atomic_numbers)
+        n = len(atomic_numbers)
+        if sigma_matrix.shape != (n, n):
+            raise ValueError(f"sigma_matrix must have shape ({n}, {n})")
+        if epsilon_matrix.shape != (n, n):
+            raise ValueError(f"epsilon_matrix must have shape ({n}, {n})")
+        if alpha_matrix is not None and alpha_matrix.shape != (n, n):
+            raise ValueError(f"alpha_matrix must have shape ({n}, {n})")
+        self.register_buffer("atomic_numbers", atomic_numbers)
+        self.sigma_matrix = sigma_matrix
+        self.epsilon_matrix = epsilon_matrix
+        self.alpha_matrix = (alpha_matrix if alpha_matrix is not None
+                             else torch.full((n, n), alpha_default, dtype=sigma_matrix.dtype,
+                                             device=sigma_matrix.device))
+        z_to_idx = torch.full((int(atomic_numbers.max().item()) + 1,), -1, dtype=torch.long)
+        for idx, z in enumerate(atomic_numbers.tolist()):
+            z_to_idx[int(z)] = idx
+        self.register_buffer("z_to_idx", z_to_idx)
+
+    def _lookup(self, zi, zj):
+        a, b = self.z_to_idx[zi], self.z_to_idx[zj]
+        return self.sigma_matrix[a, b], self.epsilon_matrix[a, b], self.alpha_matrix[a, b]
+
+
+class MultiSoftSpherePairFn(_SpeciesPairFn):
+    """Species-pair soft sphere (soft_sphere.py:90-167)."""
+
+    def forward(self, dr, zi, zj):
+        sigma, epsilon, alpha = self._lookup(zi, zj)
+        energy = epsilon / alpha * (1.0 - dr / sigma).pow(alpha)
+        return torch.where(dr < sigma, energy, torch.zeros_like(energy))
+
+
+class MultiLennardJonesPairFn(_SpeciesPairFn):
+    """Species-pair LJ 12-6 (sigma, epsilon matrices).  The reference has no
+    such class; it is the LJ counterpart of MultiSoftSpherePairFn for binary
+    LJ mixtures (alpha is unused)."""
+
+    def __init__(self, atomic_numbers, sigma_matrix, epsilon_matrix):
+        super().__init__(atomic_numbers, sigma_matrix, epsilon_matrix,
+                         torch.zeros_like(sigma_matrix))
+
+    def forward(self, dr, zi, zj):
+        sigma, epsilon, _ = self._lookup(zi, zj)
+        idr6 = (sigma / dr).pow(6)
+        energy = 4.0 * epsilon * (idr6 * idr6 - idr6)
+        return torch.where(dr > 0, energy, torch.zeros_like(energy))
+
+
+_CLOSED_FORMS = {lennard_jones_pair: LJ, morse_pair: MORSE, soft_sphere_pair: SOFT_SPHERE}
+
+
+def resolve_pair_fn(pair_fn):
+    """(kind, sigma, epsilon, alpha, species_z) of a recognised pair function.
+
+    Scalars come back as given (floats or 0-d tensors, which may require
+    grad); species-pair functions give [n, n] matrices and the atomic numbers
+    of their rows."""
+    if isinstance(pair_fn, (MultiSoftSpherePairFn, MultiLennardJonesPairFn)):
+        kind = SOFT_SPHERE if isinstance(pair_fn, MultiSoftSpherePairFn) else LJ
+        return (kind, pair_fn.sigma_matrix, pair_fn.epsilon_matrix, pair_fn.alpha_matrix,
+                pair_fn.atomic_numbers)
+    func, kw = pair_fn, {}
+    if isinstance(pair_fn, functools.partial):
+        func, kw = pair_fn.func, dict(pair_fn.keywords)
+        if pair_fn.args:
+            func = None  # positional binding of dr / zi / zj: not a closed form here
+    kind = _CLOSED_FORMS.get(func) if func is not None else None
+    if kind is None:
+        raise TypeError(
+            "torch_sim_amd evaluates pair potentials in HIP kernels and recognises "
+            "functools.partial(lennard_jones_pair | morse_pair | soft_sphere_pair, sigma=..., "
+            "epsilon=..., alpha=...) and MultiSoftSpherePairFn / MultiLennardJonesPairFn; "
+            f"got {pair_fn!r}"
+        )
+    sig = inspect.signature(func).parameters
+    allowed = {"sigma", "epsilon"} | ({"alpha"} if kind != LJ else set())
+    extra = set(kw) - allowed
+    if extra:
+        raise TypeError(f"unknown parameters {sorted(extra)} for {func.__name__}")
+    vals = {k: kw.get(k, sig[k].default) for k in allowed}
+    return kind, vals["sigma"], vals["epsilon"], vals.get("alpha", 2.0), None
 
 
 class ModelInterface(torch.nn.Module):
@@ -44,8 +189,8 @@ class ModelInterface(torch.nn.Module):
     @device.setter
     def device(self, device: torch.device) -> None:
         raise NotImplementedError(
-            "No device setter has been defined for this model"
-            " so the device cannot be changed after initialization."
+            f"{type(self).__name__} fixes its device at construction; build a new model "
+            "for another device"
         )
 
     @property
@@ -55,8 +200,8 @@ class ModelInterface(torch.nn.Module):
     @dtype.setter
     def dtype(self, dtype: torch.dtype) -> None:
         raise NotImplementedError(
-            "No dtype setter has been defined for this model"
-            " so the dtype cannot be changed after initialization."
+            f"{type(self).__name__} fixes its dtype at construction; build a new model "
+            "for another dtype"
         )
 
     @property
@@ -66,8 +211,7 @@ class ModelInterface(torch.nn.Module):
     @compute_stress.setter
     def compute_stress(self, compute_stress: bool) -> None:
         raise NotImplementedError(
-            "No compute_stress setter has been defined for this model"
-            " so compute_stress cannot be set after initialization."
+            f"{type(self).__name__} fixes compute_stress at construction"
         )
 
     @property
@@ -77,8 +221,7 @@ class ModelInterface(torch.nn.Module):
     @compute_forces.setter
     def compute_forces(self, compute_forces: bool) -> None:
         raise NotImplementedError(
-            "No compute_forces setter has been defined for this model"
-            " so compute_forces cannot be set after initialization."
+            f"{type(self).__name__} fixes compute_forces at construction"
         )
 
     @property
@@ -95,25 +238,40 @@ def _default_device(device) -> torch.device:
     return torch.device(device)
 
 
-class PairPotentialModel(ModelInterface):
-    """Closed-form pair potential evaluated by the HIP pair kernel.
+def _check_dtype(state, dtype) -> None:
+    if state.dtype != dtype:
+        raise TypeError(
+            f"SimState dtype {state.dtype} does not match model dtype {dtype}. "
+            f"Either set the model dtype to {state.dtype} or convert the SimState "
+            f"to {dtype} using sim_state.to(dtype={dtype})."
+        )
 
-    ``kind`` is one of LJ / MORSE / SOFT_SPHERE; ``sigma``, ``epsilon`` and
-    ``alpha`` are symmetric [n_species, n_species] matrices (or scalars for a
-    single species).  ``species_z`` lists the atomic numbers of the species
-    rows; atoms are mapped to rows through a lookup as in
-    MultiSoftSpherePairFn (soft_sphere.py:90-167).
+
+def _check_nl_fn(neighbor_list_fn) -> None:
+    if neighbor_list_fn is not None and neighbor_list_fn not in _NL_FNS:
+        raise NotImplementedError(
+            "torch_sim_amd models build their neighbor list with the HIP cell list "
+            "(torchsim_nl and its aliases torch_nl_linked_cell, torch_nl_n2, "
+            f"default_batched_nl); got {neighbor_list_fn!r}"
+        )
+
+
+class PairPotentialModel(ModelInterface):
+    """Batched pair potential (pair_potential.py:277-464) on the HIP kernels.
+
+    ``pair_fn`` must be one of the recognised closed forms (module
+    docstring); its parameters may be floats, tensors or, for species-pair
+    functions, symmetric [n, n] matrices.  Extra keywords of this backend:
+    ``tpa`` (lanes per atom in the pair kernel) and ``skin`` (keep a Verlet
+    list between calls; results equal a rebuild at every call).  ``device``
+    defaults to the current GPU (there is no CPU path).
     """
 
     def __init__(
         self,
-        kind: int,
+        pair_fn,
         *,
-        sigma,
-        epsilon,
-        alpha=2.0,
         cutoff: float,
-        species_z=None,
         device: torch.device | None = None,
         dtype: torch.dtype = torch.float64,
         compute_forces: bool = True,
@@ -121,11 +279,13 @@ class PairPotentialModel(ModelInterface):
         per_atom_energies: bool = False,
         per_atom_stresses: bool = False,
         neighbor_list_fn=torchsim_nl,
-        reduce_to_half_list: bool = True,
+        reduce_to_half_list: bool = False,
         retain_graph: bool = False,
         tpa: int = 4,
         skin: float = 0.0,
     ) -> None:
+        kind, sigma, epsilon, alpha, species_z = resolve_pair_fn(pair_fn)
+        _check_nl_fn(neighbor_list_fn)
         super().__init__()
         self._device = _default_device(device)
         self._dtype = dtype
@@ -133,55 +293,70 @@ class PairPotentialModel(ModelInterface):
         self._compute_stress = compute_stress
         self.per_atom_energies = per_atom_energies
         self.per_atom_stresses = per_atom_stresses
-        if neighbor_list_fn is not torchsim_nl and neighbor_list_fn is not None:
-            raise NotImplementedError(
-                "torch_sim_amd models build their neighbor list with the HIP cell list; "
-                "a custom neighbor_list_fn is not supported"
-            )
+        self.pair_fn = pair_fn
         self.neighbor_list_fn = torchsim_nl
         self.reduce_to_half_list = reduce_to_half_list
         self.retain_graph = retain_graph
         self.kind = int(kind)
         self.cutoff = float(cutoff)
-        sig = torch.as_tensor(sigma, dtype=torch.float64)
-        eps = torch.as_tensor(epsilon, dtype=torch.float64)
-        alp = torch.as_tensor(alpha, dtype=torch.float64)
-        if sig.ndim == 0:
-            n = 1
-            sig, eps, alp = sig.view(1, 1), eps.view(1, 1), alp.view(1, 1)
-        else:
-            n = sig.shape[0]
-            eps = eps.expand(n, n) if eps.ndim == 0 else eps
-            alp = alp.expand(n, n) if alp.ndim == 0 else alp
-        for name, m in (("sigma", sig), ("epsilon", eps), ("alpha", alp)):
-            if m.shape != (n, n):
-                raise ValueError(f"{name} matrix must have shape ({n}, {n})")
-            if not torch.allclose(m, m.T):
-                raise ValueError(f"{name}_matrix is not symmetric")
+        self._param_src = (sigma, epsilon, alpha)
+        self._param_versions = None
+        sig = self._matrix(sigma)
+        self.n_species = n = sig.shape[0]
         if n > 8:
             raise ValueError("at most 8 species are supported")
-        self.n_species = n
-        self._params = (
-            torch.stack([sig, eps, alp], dim=-1).reshape(n * n, 3).contiguous().to(self._device)
-        )
+        self._refresh_params()
         self._z_to_idx = None
         if n > 1:
             if species_z is None:
-                raise ValueError("species_z (atomic numbers of the species) is required")
-            z = torch.as_tensor(species_z, dtype=torch.long).flatten()
+                raise ValueError("species-pair parameters need the atomic numbers of the rows")
+            z = torch.as_tensor(species_z, dtype=torch.long).flatten().cpu()
             table = torch.full((int(z.max()) + 1,), -1, dtype=torch.int32)
             for k, zz in enumerate(z.tolist()):
                 table[int(zz)] = k
             self._z_to_idx = table.to(self._device)
         self._nl = NeighborList(self.cutoff, tpa=tpa)
-        # skin > 0: keep a Verlet list between calls and rebuild only the systems
-        # that left the skin (results equal a rebuild at every call)
         self.skin = float(skin)
         self._vl = VerletList(self.cutoff, self.skin, tpa=tpa) if self.skin > 0 else None
         self._sysidx: SystemIndex | None = None
         self._species_key = None
         self._species = None
+        self._diff_terms = None
         self.tpa = tpa
+
+    # ------------------------------------------------------------- params
+    @staticmethod
+    def _matrix(v) -> torch.Tensor:
+        t = torch.as_tensor(v).detach().to(torch.float64).cpu()
+        return t.view(1, 1) if t.ndim == 0 else t
+
+    def _refresh_params(self) -> None:
+        """The [n^2, 3] device table of (sigma, epsilon, alpha), rebuilt when a
+        parameter tensor changed in place since the last build."""
+        versions = tuple(v._version if isinstance(v, torch.Tensor) else None
+                         for v in self._param_src)
+        if versions == self._param_versions:
+            return
+        sig, eps, alp = (self._matrix(v) for v in self._param_src)
+        n = sig.shape[0]
+        eps = eps.expand(n, n) if eps.numel() == 1 else eps
+        alp = alp.expand(n, n) if alp.numel() == 1 else alp
+        for name, m in (("sigma", sig), ("epsilon", eps), ("alpha", alp)):
+            if m.shape != (n, n):
+                raise ValueError(f"{name} matrix must have shape ({n}, {n})")
+            if not torch.allclose(m, m.T):
+                raise ValueError(f"{name}_matrix is not symmetric")
+        self._params = torch.stack([sig, eps, alp], dim=-1).reshape(n * n, 3).contiguous().to(
+            self._device)
+        self._param_versions = versions
+
+    def graph_params(self) -> torch.Tensor:
+        """[n, n, 3] (sigma, epsilon, alpha) built from the pair function's own
+        values, so tensors among them that require grad stay in the graph."""
+        from .autograd import pair_params
+
+        sig, eps, alp = self._param_src
+        return pair_params(sig, eps, alp).to(self._device)
 
     # ------------------------------------------------------------- helpers
     def _system_index(self, state) -> SystemIndex:
@@ -207,18 +382,23 @@ class PairPotentialModel(ModelInterface):
         self._species_key, self._species = TensorKey(atomic_numbers), sp
         return sp
 
+    def _attached(self, state, out) -> None:
+        """Replace energy / forces by outputs attached to autograd."""
+        from . import autograd as tad
+
+        if self._diff_terms is None:
+            self._diff_terms = tad.PairTerms(self)
+        geo = tad.geometry(self._diff_terms, state)
+        params = self.graph_params()
+        dt = self._dtype
+        out["energy"] = tad.pair_energy(state.positions, state.cell, params, geo).to(dt)
+        if self._compute_forces:
+            out["forces"] = tad.pair_forces(state.positions, state.cell, params, geo).to(dt)
+
     def forward(self, state, **_kwargs) -> dict[str, torch.Tensor]:
-        if state.dtype != self._dtype:
-            raise TypeError(
-                f"SimState dtype {state.dtype} does not match model dtype {self._dtype}. "
-                f"Either set the model dtype to {state.dtype} or convert the SimState "
-                f"to {self._dtype} using sim_state.to(dtype={self._dtype})."
-            )
-        if self.retain_graph:
-            raise NotImplementedError(
-                "retain_graph=True: use torch_sim_amd.autograd for differentiable pair terms"
-            )
+        _check_dtype(state, self._dtype)
         _lib.require_gpu(state.positions, "state.positions")
+        self._refresh_params()
         pos = state.positions.detach().to(torch.float64).contiguous()
         cell = state.cell.detach().to(torch.float64).contiguous()
         n, s = state.n_atoms, state.n_systems
@@ -272,6 +452,8 @@ class PairPotentialModel(ModelInterface):
             out["stress"] = stress.to(dt)
             if stresses is not None:
                 out["stresses"] = stresses.to(dt)
+        if self.retain_graph and n:
+            self._attached(state, out)
         return out
 
 
@@ -298,10 +480,8 @@ class LennardJonesModel(PairPotentialModel):
         self.sigma = sigma
         self.epsilon = epsilon
         super().__init__(
-            LJ,
-            sigma=sigma,
-            epsilon=epsilon,
-            cutoff=cutoff if cutoff is not None else 2.5 * sigma,
+            pair_fn=functools.partial(lennard_jones_pair, sigma=sigma, epsilon=epsilon),
+            cutoff=cutoff if cutoff is not None else 2.5 * float(sigma),
             device=device,
             dtype=dtype,
             compute_forces=compute_forces,
@@ -309,6 +489,7 @@ class LennardJonesModel(PairPotentialModel):
             per_atom_energies=per_atom_energies,
             per_atom_stresses=per_atom_stresses,
             neighbor_list_fn=neighbor_list_fn,
+            reduce_to_half_list=True,
             retain_graph=retain_graph,
             tpa=tpa,
             skin=skin,
@@ -340,11 +521,8 @@ class MorseModel(PairPotentialModel):
         self.epsilon = epsilon
         self.alpha = alpha
         super().__init__(
-            MORSE,
-            sigma=sigma,
-            epsilon=epsilon,
-            alpha=alpha,
-            cutoff=cutoff if cutoff is not None else 2.5 * sigma,
+            pair_fn=functools.partial(morse_pair, sigma=sigma, epsilon=epsilon, alpha=alpha),
+            cutoff=cutoff if cutoff is not None else 2.5 * float(sigma),
             device=device,
             dtype=dtype,
             compute_forces=compute_forces,
@@ -352,6 +530,7 @@ class MorseModel(PairPotentialModel):
             per_atom_energies=per_atom_energies,
             per_atom_stresses=per_atom_stresses,
             neighbor_list_fn=neighbor_list_fn,
+            reduce_to_half_list=True,
             retain_graph=retain_graph,
             tpa=tpa,
             skin=skin,
@@ -384,11 +563,9 @@ class SoftSphereModel(PairPotentialModel):
         self.epsilon = epsilon
         self.alpha = alpha
         super().__init__(
-            SOFT_SPHERE,
-            sigma=sigma,
-            epsilon=epsilon,
-            alpha=alpha,
-            cutoff=cutoff if cutoff is not None else sigma,
+            pair_fn=functools.partial(soft_sphere_pair, sigma=sigma, epsilon=epsilon,
+                                      alpha=alpha),
+            cutoff=cutoff if cutoff is not None else float(sigma),
             device=device,
             dtype=dtype,
             compute_forces=compute_forces,
@@ -396,10 +573,17 @@ class SoftSphereModel(PairPotentialModel):
             per_atom_energies=per_atom_energies,
             per_atom_stresses=per_atom_stresses,
             neighbor_list_fn=neighbor_list_fn,
+            reduce_to_half_list=True,
             retain_graph=retain_graph,
             tpa=tpa,
             skin=skin,
         )
+
+
+def _species_matrix(m, n, default) -> torch.Tensor:
+    if m is None:
+        return torch.full((n, n), default, dtype=torch.float64)
+    return m
 
 
 class SoftSphereMultiModel(PairPotentialModel):
@@ -430,21 +614,18 @@ class SoftSphereMultiModel(PairPotentialModel):
         self.use_neighbor_list = use_neighbor_list
         unique_z = torch.unique(torch.as_tensor(atomic_numbers).cpu()).sort().values.long()
         n = len(unique_z)
-        ones = torch.ones((n, n), dtype=torch.float64)
-        sig = sigma_matrix.detach().cpu().double() if sigma_matrix is not None else ones
-        eps = epsilon_matrix.detach().cpu().double() if epsilon_matrix is not None else ones
-        alp = alpha_matrix.detach().cpu().double() if alpha_matrix is not None else 2.0 * ones
+        sig = _species_matrix(sigma_matrix, n, 1.0)
+        eps = _species_matrix(epsilon_matrix, n, 1.0)
+        alp = _species_matrix(alpha_matrix, n, 2.0)
         for name, m in (("sigma_matrix", sig), ("epsilon_matrix", eps), ("alpha_matrix", alp)):
             if m.shape != (n, n):
                 raise ValueError(f"{name} must have shape ({n}, {n})")
+            if not torch.allclose(m.detach().cpu().double(), m.detach().cpu().double().T):
+                raise ValueError(f"{name} is not symmetric")
         self.sigma_matrix, self.epsilon_matrix, self.alpha_matrix = sig, eps, alp
         super().__init__(
-            SOFT_SPHERE,
-            sigma=sig,
-            epsilon=eps,
-            alpha=alp,
-            cutoff=cutoff or float(sig.max()),
-            species_z=unique_z,
+            pair_fn=MultiSoftSpherePairFn(unique_z, sig, eps, alp),
+            cutoff=cutoff or float(sig.detach().max()),
             device=device,
             dtype=dtype,
             compute_forces=compute_forces,
@@ -452,6 +633,7 @@ class SoftSphereMultiModel(PairPotentialModel):
             per_atom_energies=per_atom_energies,
             per_atom_stresses=per_atom_stresses,
             neighbor_list_fn=neighbor_list_fn,
+            reduce_to_half_list=True,
             retain_graph=retain_graph,
             tpa=tpa,
             skin=skin,
@@ -463,7 +645,7 @@ class MultiLennardJonesModel(PairPotentialModel):
 
     The reference has no multi-species LJ class; its counterpart is a
     PairPotentialModel with a species-lookup pair function in the style of
-    MultiSoftSpherePairFn (soft_sphere.py:90-167).
+    MultiSoftSpherePairFn (soft_sphere.py:90-167), which is what this builds.
     """
 
     def __init__(
@@ -479,25 +661,24 @@ class MultiLennardJonesModel(PairPotentialModel):
         compute_stress: bool = False,
         per_atom_energies: bool = False,
         per_atom_stresses: bool = False,
+        retain_graph: bool = False,
         tpa: int = 4,
         skin: float = 0.0,
     ) -> None:
         unique_z = torch.unique(torch.as_tensor(atomic_numbers).cpu()).sort().values.long()
-        self.sigma_matrix = sigma_matrix.detach().cpu().double()
-        self.epsilon_matrix = epsilon_matrix.detach().cpu().double()
+        self.sigma_matrix = sigma_matrix
+        self.epsilon_matrix = epsilon_matrix
         super().__init__(
-            LJ,
-            sigma=self.sigma_matrix,
-            epsilon=self.epsilon_matrix,
-            alpha=torch.zeros_like(self.sigma_matrix),
+            pair_fn=MultiLennardJonesPairFn(unique_z, sigma_matrix, epsilon_matrix),
             cutoff=cutoff,
-            species_z=unique_z,
             device=device,
             dtype=dtype,
             compute_forces=compute_forces,
             compute_stress=compute_stress,
             per_atom_energies=per_atom_energies,
             per_atom_stresses=per_atom_stresses,
+            reduce_to_half_list=True,
+            retain_graph=retain_graph,
             tpa=tpa,
             skin=skin,
         )
