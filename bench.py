@@ -1,17 +1,31 @@
 """Benchmark: MD atom-steps/s on one MI355X, batched (graph-captured) vs plain.
 
-Workload (default): 512 periodic Cu fcc systems with random repeats (32-2048
-atoms each, ~4.0e5 atoms), positions jittered N(0, 0.05 A), Morse
+Headline workload: 512 periodic Cu fcc systems with random repeats (32-2048
+atoms each, 402,324 atoms), positions jittered N(0, 0.05 A), Morse
 D = 0.3429 eV, alpha = 1.3588 1/A, r0 = 2.866 A, cutoff 6 A, Langevin NVT
 T = 1000 K, gamma = 10 / ps, dt = 2 fs.  FP32 pair arithmetic, FP64
 accumulation / integration, synthetic seeded data.
 
-Legs reported on one JSON line:
-  value        batched fused engine, hipGraph replay, K timed steps
-  plain_gpu    the per-step path (nvt_langevin_step + model(state) with a
-               neighbor list rebuilt every step) on the same GPU
-  roofline     fused-step kernel: algorithmic bytes / measured duration
-  cpu_baseline the FP64 CPU oracle (reference algorithm) on a bounded sample
+One JSON line:
+  value          the fused, graph-replayed engine over exactly --steps timed
+                 steps, after max(--warmup, 200) untimed ones (the rebuild
+                 rate settles at about 40 systems per step after ~150 steps
+                 from the lattice start; the window before that is faster)
+  steady_1000    1000 more timed steps of the same run, with rebuilds / step
+  run_1000       a fresh engine timed over the workload's own 1000-step run
+                 from the initial state (list build and capture included)
+  plain_gpu      the reference pipeline in PyTorch ops on the GPU (the
+                 denominator of the >= 5x target); per_step_hip beside it
+  roofline       the fused-step kernel: the survey's algorithmic bytes
+                 (184 + 4 k_list per atom-step) over its measured duration;
+                 this design's own extra bytes reported apart
+                 (overhead_bytes_per_atom) and the counter-measured traffic
+                 with its source
+  cpu_baseline   the FP64 CPU oracle on the host cores: a truncated run of
+                 the full 512-system batch (value) and the full Ar-256 NVE run
+  legs           the other configurations: 1M-atom LJ NVE, soft-sphere FIRE +
+                 Frechet relaxed to fmax 1e-4 with in-place refill from a
+                 queue, and the binary-LJ differentiable fit (grad-evals/s)
 """
 
 from __future__ import annotations
@@ -20,7 +34,7 @@ import argparse
 import json
 import math
 import os
-import platform
+import subprocess
 import sys
 import time
 
@@ -33,44 +47,97 @@ import torch_sim_amd as tsa  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 MORSE = dict(sigma=2.866, epsilon=0.3429, alpha=1.3588)
+WARMUP_FLOOR = 200
 
-# HBM bytes per dispatch of the fused-step kernel, from two rocprofv3 --pmc runs of this
-# script (FETCH_SIZE, WRITE_SIZE in KiB per dispatch, averaged over the dispatches), each
-# scaled by what the same counter reads for a 256 MiB device copy (FETCH_SIZE x 2.000,
-# WRITE_SIZE x 1.000).  Keyed by (workload, skin, tpa); the summaries with their commands
-# are profiles/k_md_fused__cu_morse_nvt_512sys.md and profiles/k_md_fused__lj_fluid_1m_nve.md.
+# Algorithmic bytes per atom-step of the fused step: the integrator reads x,
+# p, F (3 x 24) and m (8) and writes x, p (2 x 24) and a packed FP32 copy of
+# the position (16); the pair pass reads its own packed position (16), one
+# 4-byte list word per stored neighbour and writes F (24).  The engine moves
+# 84 more bytes per atom-step of its own design: the bin anchor (32), the
+# displacement reference of the skin test (32), row order / system / row
+# length (12) and the per-atom energy read back by the reduction (8).
+ALGO_BYTES_BASE = 184.0
+OVERHEAD_BYTES = 84.0
+
+# HBM bytes per dispatch of the fused-step kernel from two rocprofv3 --pmc runs
+# (FETCH_SIZE, WRITE_SIZE in KiB per dispatch, averaged over the dispatches),
+# each scaled by what the same counter reads for a 256 MiB device copy
+# (bench.py --calibrate-copy 256: FETCH_SIZE x 2.000, WRITE_SIZE x 1.000).
+# Keyed by (workload, skin, tpa); "source" names the summary with the commands.
 MEASURED_TRAFFIC = {
     ("cu_morse_nvt", 0.6, 2): {"fetch_kib": 111967.1, "fetch_factor": 2.000,
-                               "write_kib": 41957.1, "write_factor": 1.000},
+                               "write_kib": 41957.1, "write_factor": 1.000,
+                               "source": "profiles/k_md_fused__cu_morse_nvt_512sys.md"},
     ("cu_morse_nvt", 0.66, 2): {"fetch_kib": 117702.1, "fetch_factor": 2.000,
-                                "write_kib": 41769.9, "write_factor": 1.000},
+                                "write_kib": 41769.9, "write_factor": 1.000,
+                                "source": "profiles/k_md_fused__cu_morse_nvt_512sys.md"},
     ("cu_morse_nvt", 0.72, 2): {"fetch_kib": 123122.7, "fetch_factor": 2.000,
-                                "write_kib": 41832.4, "write_factor": 1.000},
+                                "write_kib": 41832.4, "write_factor": 1.000,
+                                "source": "profiles/k_md_fused__cu_morse_nvt_512sys.md"},
     ("lj1m_nve", 0.3, 4): {"fetch_kib": 342936.1, "fetch_factor": 2.000,
-                           "write_kib": 125831.8, "write_factor": 1.000},
+                           "write_kib": 125831.8, "write_factor": 1.000,
+                           "source": "profiles/k_md_fused__lj_fluid_1m_nve.md"},
 }
 
 
 def measured_traffic(workload: str, skin: float, tpa: int):
     meas = MEASURED_TRAFFIC.get((workload, round(skin, 3), tpa))
     if meas is None:
-        return None
-    return 1024.0 * (meas["fetch_kib"] * meas["fetch_factor"] + meas["write_kib"] * meas["write_factor"])
+        return None, None
+    b = 1024.0 * (meas["fetch_kib"] * meas["fetch_factor"]
+                  + meas["write_kib"] * meas["write_factor"])
+    return b, meas["source"]
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_engine(state, model, skin, tpa, seed=1):
+def host_cpu() -> dict:
+    """CPU model, physical cores and the threads this process may use (the
+    affinity mask, capped by OMP_NUM_THREADS when set)."""
+    model, sockets, cores_per = None, 1, None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            v = v.strip()
+            if k.strip() == "Model name":
+                model = v
+            elif k.strip() == "Socket(s)":
+                sockets = int(v)
+            elif k.strip() == "Core(s) per socket":
+                cores_per = int(v)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = min(avail, int(omp)) if omp and omp.isdigit() else avail
+    phys = sockets * cores_per if cores_per else None
+    if phys:
+        threads = min(threads, phys)
+    return {"model": model, "physical_cores": phys, "cpus_available": avail,
+            "threads": threads, "omp_num_threads": omp}
+
+
+def make_engine(state, model, skin, tpa, bins=2, seed=1):
     kT = 1000.0 * tsa.MetalUnits.temperature
     dt = 0.002 * tsa.MetalUnits.time
     gamma = 10.0 / tsa.MetalUnits.time
     state.rng = seed
     md = tsa.nvt_langevin_init(state=state, model=model, kT=kT)
     eng = tsa.BatchedMD(model, md, dt=dt, kT=kT, gamma=gamma, langevin=True, skin=skin, tpa=tpa,
-                        seed=seed, chunk_steps=500)
+                        seed=seed, chunk_steps=500, bins_per_rlist=bins)
     return md, eng
+
+
+def timed_run(eng, steps: int):
+    torch.cuda.synchronize()
+    r0 = eng.n_rebuilds
+    t0 = time.perf_counter()
+    eng.run(steps)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, eng.n_rebuilds - r0
 
 
 def time_fused_kernel(eng, n_launch=50, n_warm=5):
@@ -81,7 +148,9 @@ def time_fused_kernel(eng, n_launch=50, n_warm=5):
     of every launch keeps the host ahead of the GPU, so each event pair
     brackets the kernel alone, in the cache state of the real step."""
     st = torch.cuda.current_stream()
-    eng._open()
+    if not eng._is_open:
+        eng._open()
+        eng._is_open = True
     pairs = []
     for k in range(n_warm + n_launch):
         q = eng.cur
@@ -98,6 +167,29 @@ def time_fused_kernel(eng, n_launch=50, n_warm=5):
             pairs.append((a, b))
     torch.cuda.synchronize()
     return sum(a.elapsed_time(b) for a, b in pairs) / len(pairs)
+
+
+def roofline(eng, n_atoms, workload, skin, tpa) -> dict:
+    """The fused kernel against HBM peak with the survey's byte model."""
+    k = eng.mean_neighbors
+    ms = time_fused_kernel(eng)
+    algo = (ALGO_BYTES_BASE + 4.0 * k) * n_atoms
+    achieved = algo / (ms * 1e-3) / 1e9
+    traffic, source = measured_traffic(workload, skin, tpa)
+    log(f"fused kernel: {ms * 1e3:.1f} us, k_list {k:.1f}, {achieved:.0f} GB/s algorithmic "
+        f"({achieved / HBM_PEAK_GBS:.3f} of peak)")
+    return {
+        "bound": "hbm", "kernel": "k_md_fused (tsx_md_fused_step)", "achieved": achieved,
+        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+        "traffic": traffic,
+        "algorithmic_bytes_per_atom": ALGO_BYTES_BASE + 4.0 * k,
+        "bytes_per_launch": algo, "mean_list_neighbors": k, "kernel_ms": ms,
+        "overhead_bytes_per_atom": OVERHEAD_BYTES,
+        "with_overhead_frac": (algo + OVERHEAD_BYTES * n_atoms) / (ms * 1e-3) / 1e9
+        / HBM_PEAK_GBS,
+        "traffic_over_algorithmic": None if traffic is None else traffic / algo,
+        "traffic_source": source,
+    }
 
 
 def plain_gpu(state, model, steps, warmup):
@@ -221,54 +313,98 @@ def plain_torch(state, steps, warmup):
     return state.n_atoms * steps / (time.perf_counter() - t0)
 
 
-def cpu_baseline(n_sys=32, min_seconds=10.0, max_steps=200, seed=0):
-    """FP64 CPU oracle (reference algorithm: linked-cell list rebuilt every
-    step, autograd half-list forces, BAOAB) on the first ``n_sys`` systems of
-    the same workload, stepped until ``min_seconds`` of CPU work have run (a
-    bounded sample of the workload)."""
-    from oracle import md as omd
+
+def _oracle_force_chunked(st, fn, cutoff, chunk=128):
+    """Oracle forces / energies of a batch in groups of ``chunk`` systems
+    (systems are independent; this bounds the oracle's candidate tensors)."""
     from oracle import models as om
     from oracle import nl as onl
 
-    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
-    torch.set_num_threads(cores)
-    st = tsa.workloads.cu_batch(512, device="cpu", seed=seed)
-    cnt = torch.bincount(st.system_idx)
-    n_atoms = int(cnt[:n_sys].sum())
-    pos = st.positions[:n_atoms].clone()
-    idx = st.system_idx[:n_atoms].clone()
-    cell = st.cell[:n_sys].clone()
-    m = st.masses[:n_atoms].clone()
-    z = st.atomic_numbers[:n_atoms].clone()
-    pbc = st.pbc
-    fn = lambda d, zi, zj: om.morse_pair(d, zi, zj, **MORSE)  # noqa: E731
+    counts = torch.bincount(st.system_idx, minlength=st.n_systems).tolist()
+    starts = [0]
+    for c in counts:
+        starts.append(starts[-1] + c)
 
     def force(x):
-        o = om.pair_forward(x, cell, pbc, z, idx, fn, 6.0, compute_stress=False, nl=onl.linked_cell_nl)
-        return o["energy"], o["forces"]
+        es, fs = [], []
+        for s0 in range(0, st.n_systems, chunk):
+            s1 = min(st.n_systems, s0 + chunk)
+            a, b = starts[s0], starts[s1]
+            o = om.pair_forward(x[a:b], st.cell[s0:s1], st.pbc, st.atomic_numbers[a:b],
+                                st.system_idx[a:b] - s0, fn, cutoff, compute_stress=False,
+                                nl=onl.linked_cell_nl)
+            es.append(o["energy"])
+            fs.append(o["forces"])
+        return torch.cat(es), torch.cat(fs)
 
+    return force
+
+
+def cpu_baseline(cfg2_steps: int = 2) -> dict:
+    """The FP64 CPU oracle (the reference algorithm: linked-cell list rebuilt
+    every step, autograd half-list forces, BAOAB / velocity Verlet) on the
+    host cores: ``cfg2_steps`` BAOAB steps of the full 512-system Cu batch
+    after one untimed force call (a truncated run of the headline workload:
+    ``value``), and the full Ar-256 NVE run of 100 steps after one untimed
+    step."""
+    from oracle import md as omd
+    from oracle import models as om
+
+    cpu = host_cpu()
+    torch.set_num_threads(cpu["threads"])
+    st = tsa.workloads.cu_batch(512, device="cpu", seed=0)
+    n = st.n_atoms
+    force = _oracle_force_chunked(st, lambda d, zi, zj: om.morse_pair(d, zi, zj, **MORSE), 6.0)
     kT = 1000.0 * omd.KB_EV
     dt = 0.002 * omd.TIME_UNIT
     gamma = 10.0 / omd.TIME_UNIT
     g = torch.Generator().manual_seed(0)
-    p = omd.initialize_momenta(m, idx, torch.tensor(kT, dtype=torch.float64), torch.randn(n_atoms, 3, generator=g, dtype=torch.float64))
-    e, f = force(pos)  # warm-up / init
-    x = pos
+    p = omd.initialize_momenta(st.masses, st.system_idx, torch.tensor(kT, dtype=torch.float64),
+                               torch.randn(n, 3, generator=g, dtype=torch.float64))
     t0 = time.perf_counter()
-    steps = 0
-    while steps < max_steps and (steps < 2 or time.perf_counter() - t0 < min_seconds):
-        noise = torch.randn(n_atoms, 3, generator=g, dtype=torch.float64)
-        x, p, f, e = omd.langevin_step(x, p, f, m, idx, force, dt, kT, gamma, noise)
-        steps += 1
+    e, f = force(st.positions)
+    t_first = time.perf_counter() - t0
+    x = st.positions
+    t0 = time.perf_counter()
+    for _ in range(cfg2_steps):
+        noise = torch.randn(n, 3, generator=g, dtype=torch.float64)
+        x, p, f, e = omd.langevin_step(x, p, f, st.masses, st.system_idx, force, dt, kT, gamma,
+                                       noise)
     el = time.perf_counter() - t0
+    value = n * cfg2_steps / el
+    log(f"cpu baseline (512-system batch, {cfg2_steps} steps): {value:.4e} atom-steps/s")
+
+    ar = tsa.workloads.ar_fcc(device="cpu")
+    fn = lambda d, zi, zj: om.lennard_jones_pair(d, zi, zj, 3.405, 0.0104)  # noqa: E731
+    ar_force = _oracle_force_chunked(ar, fn, 8.5125)
+    g = torch.Generator().manual_seed(0)
+    pa = omd.initialize_momenta(ar.masses, ar.system_idx, torch.tensor(50.0 * omd.KB_EV),
+                                torch.randn(256, 3, generator=g, dtype=torch.float64))
+    xa = ar.positions.clone()
+    ea, fa = ar_force(xa)
+    dta = 0.002 * omd.TIME_UNIT
+    xa, pa, fa, ea = omd.nve_step(xa, pa, fa, ar.masses, ar_force, dta)  # untimed
+    t0 = time.perf_counter()
+    for _ in range(100):
+        xa, pa, fa, ea = omd.nve_step(xa, pa, fa, ar.masses, ar_force, dta)
+    ela = time.perf_counter() - t0
+    log(f"cpu baseline (Ar-256, 100 NVE steps): {256 * 100 / ela:.4e} atom-steps/s")
     return {
-        "value": n_atoms * steps / el,
+        "value": value,
         "unit": "atom-steps/s",
-        "cores": torch.get_num_threads(),
+        "cores": cpu["threads"],
         "kind": "port",
-        "sample": f"first {n_sys} systems ({n_atoms} atoms) of the 512-system batch, {steps} "
-        f"BAOAB steps, FP64 CPU oracle (linked-cell list every step, autograd forces), "
-        f"{el:.1f} s, cpu={platform.processor() or platform.machine()}",
+        "sample": f"truncated run of the headline workload: the full 512-system Cu batch "
+                  f"({n} atoms), {cfg2_steps} BAOAB steps after one untimed force call "
+                  f"({t_first:.1f} s), FP64 CPU oracle (linked-cell list every step, autograd "
+                  f"half-list forces), {el:.1f} s on {cpu['threads']} threads",
+        "cpu_model": cpu["model"],
+        "physical_cores": cpu["physical_cores"],
+        "cpus_available": cpu["cpus_available"],
+        "ar256_nve_full": {"value": 256 * 100 / ela, "unit": "atom-steps/s", "steps": 100,
+                           "seconds": ela,
+                           "sample": "Ar fcc 4x4x4 (256 atoms), LJ rc 8.5125 A, 50 K, dt 2 fs, "
+                                     "100 velocity-Verlet steps after one untimed step"},
     }
 
 
@@ -286,29 +422,30 @@ def bench_cu_morse(args) -> dict:
         cfg = autotune.tune_md(state, model, log=log)
         skin = args.skin if args.skin is not None else cfg["skin"]
         tpa = args.tpa if args.tpa is not None else cfg["tpa"]
+        bins = args.bins if args.bins is not None else cfg.get("bins_per_rlist", 2)
     else:
-        skin, tpa = args.skin, args.tpa
-    log(f"engine config: skin={skin} tpa={tpa}")
+        skin, tpa, bins = args.skin, args.tpa, args.bins or 2
+    log(f"engine config: skin={skin} tpa={tpa} bins_per_rlist={bins}")
 
-    md, eng = make_engine(state, model, skin, tpa)
-    eng.run(args.warmup)
-    torch.cuda.synchronize()
-    reb0 = eng.n_rebuilds
-    t0 = time.perf_counter()
-    eng.run(args.steps)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
+    md, eng = make_engine(state, model, skin, tpa, bins)
+    warm = max(args.warmup, WARMUP_FLOOR)
+    eng.run(warm)
+    el, n_reb = timed_run(eng, args.steps)
     value = n * args.steps / el
-    n_reb = eng.n_rebuilds - reb0
-    log(f"batched: {value:.4e} atom-steps/s ({1e3 * el / args.steps:.4f} ms/step), rebuilds {n_reb}")
-
-    k_mean = eng.mean_neighbors
-    ms_kernel = time_fused_kernel(eng)
-    bytes_launch = eng.algorithmic_bytes_per_atom_step() * n
-    achieved = bytes_launch / (ms_kernel * 1e-3) / 1e9
-    log(f"fused kernel: {ms_kernel * 1e3:.1f} us, k_list {k_mean:.1f}, {achieved:.0f} GB/s")
-
-    traffic = measured_traffic("cu_morse_nvt", skin, tpa) if args.n_systems == 512 else None
+    log(f"batched: {value:.4e} atom-steps/s ({1e3 * el / args.steps:.4f} ms/step), "
+        f"rebuilds {n_reb} ({n_reb / args.steps:.1f} per step)")
+    el_s, reb_s = timed_run(eng, 1000)
+    steady = n * 1000 / el_s
+    log(f"steady 1000 steps: {steady:.4e} atom-steps/s, {reb_s / 1000:.1f} rebuilds per step")
+    roof = roofline(eng, n, "cu_morse_nvt", skin, tpa)
+    traffic_ok = args.n_systems == 512 and bins == 2
+    if not traffic_ok:
+        roof["traffic"] = roof["traffic_over_algorithmic"] = roof["traffic_source"] = None
+    _, eng2 = make_engine(state, model, skin, tpa, bins)
+    el_r, reb_r = timed_run(eng2, 1000)
+    run1000 = n * 1000 / el_r
+    log(f"1000-step run from the initial state: {run1000:.4e} atom-steps/s")
+    del eng2
 
     plain = plain_hip = None
     if not args.no_plain:
@@ -316,18 +453,14 @@ def bench_cu_morse(args) -> dict:
         log(f"plain per-step GPU path (PyTorch ops): {plain:.4e} atom-steps/s")
         plain_hip = plain_gpu(state, model, args.plain_steps, 1)
         log(f"per-step HIP kernels, no graph, list every step: {plain_hip:.4e} atom-steps/s")
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(args.cpu_systems)
-        log(f"cpu baseline: {cpu['value']:.4e} atom-steps/s")
-
-    out = {
+    return {
         "metric": "MD atom-steps/s at 1 MI355X, batched graph-captured Morse NVT Langevin",
         "value": value,
         "unit": "atom-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "warmup_effective": warm,
         "ms_per_step": 1e3 * el / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
@@ -342,12 +475,19 @@ def bench_cu_morse(args) -> dict:
             "cutoff_A": 6.0,
             "skin_A": skin,
             "tpa": tpa,
+            "bins_per_rlist": bins,
             "dt_fs": 2.0,
             "T_K": 1000.0,
             "gamma_per_ps": 10.0,
             "list_rebuilds_in_timed_steps": n_reb,
-            "mean_list_neighbors": round(k_mean, 2),
+            "rebuilds_per_step": n_reb / args.steps,
         },
+        "steady_1000": {"value": steady, "unit": "atom-steps/s", "steps": 1000,
+                        "ms_per_step": el_s, "rebuilds_per_step": reb_s / 1000},
+        "run_1000": {"value": run1000, "unit": "atom-steps/s", "steps": 1000,
+                     "ms_per_step": el_r, "rebuilds_per_step": reb_r / 1000,
+                     "what": "a fresh engine over the workload's 1000-step run from the initial "
+                             "state (first list build and graph capture included)"},
         "plain_gpu": None if plain is None else {
             "value": plain, "unit": "atom-steps/s", "steps": args.plain_steps,
             "path": "reference pipeline in PyTorch ops on the GPU: torch linked-cell list every "
@@ -358,20 +498,8 @@ def bench_cu_morse(args) -> dict:
             "path": "nvt_langevin_step + model(state): HIP kernels per step, list rebuilt every "
                     "step, no graph capture"},
         "batched_vs_per_step_hip": None if plain_hip is None else value / plain_hip,
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_md_fused (tsx_md_fused_step)",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "bytes_per_launch": bytes_launch,
-            "kernel_ms": ms_kernel,
-            "traffic": traffic,
-        },
-        "cpu_baseline": cpu,
+        "roofline": roof,
     }
-    return out
 
 
 def _line(metric, value, unit, steps, warmup, ms, config, **extra) -> dict:
@@ -387,49 +515,31 @@ def _line(metric, value, unit, steps, warmup, ms, config, **extra) -> dict:
 
 def bench_lj1m(args) -> dict:
     """1,048,576-atom LJ fluid, NVE, reduced units (rho* 0.8442, T* 0.72, rc 2.5,
-    skin 0.3, dt 0.005 tau)."""
+    skin 0.3, dt 0.005 tau), 500 timed steps."""
     dev = torch.device("cuda:0")
     state = tsa.workloads.lj_fluid(64, device=dev, rho=0.8442, jitter=0.05, seed=0)
     model = tsa.LennardJonesModel(sigma=1.0, epsilon=1.0, cutoff=2.5, device=dev)
     state.rng = 0
     md = tsa.nve_init(state=state, model=model, kT=0.72)
-    tpa = args.tpa or 4
-    eng = tsa.BatchedMD(model, md, dt=0.005, langevin=False, skin=args.skin or 0.3, tpa=tpa,
-                        seed=1, chunk_steps=250)
+    tpa, skin = args.tpa or 4, args.skin or 0.3
+    eng = tsa.BatchedMD(model, md, dt=0.005, langevin=False, skin=skin, tpa=tpa, seed=1,
+                        chunk_steps=250)
     n = state.n_atoms
     eng.run(args.warmup)
-    torch.cuda.synchronize()
-    reb0 = eng.n_rebuilds
-    t0 = time.perf_counter()
-    eng.run(args.steps)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
+    el, n_reb = timed_run(eng, args.steps)
     value = n * args.steps / el
-    ms_kernel = time_fused_kernel(eng)
-    bytes_launch = eng.algorithmic_bytes_per_atom_step() * n
-    achieved = bytes_launch / (ms_kernel * 1e-3) / 1e9
-    log(f"lj1m: {value:.4e} atom-steps/s, fused kernel {ms_kernel * 1e3:.1f} us, {achieved:.0f} GB/s")
+    log(f"lj1m: {value:.4e} atom-steps/s")
     return _line(
         "MD atom-steps/s at 1 MI355X, 1M-atom LJ NVE", value, "atom-steps/s", args.steps,
         args.warmup, 1e3 * el / args.steps,
         {"workload": "lj_fluid_1m_nve", "n_atoms": n, "rho": 0.8442, "T": 0.72, "cutoff": 2.5,
-         "skin": args.skin or 0.3, "tpa": tpa, "dt_tau": 0.005,
-         "list_rebuilds_in_timed_steps": eng.n_rebuilds - reb0,
-         "mean_list_neighbors": round(eng.mean_neighbors, 2)},
-        roofline={"bound": "hbm", "kernel": "k_md_fused (tsx_md_fused_step)", "achieved": achieved,
-                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                  "bytes_per_launch": bytes_launch, "kernel_ms": ms_kernel,
-                  "traffic": measured_traffic("lj1m_nve", args.skin or 0.3, tpa)},
+         "skin": skin, "tpa": tpa, "dt_tau": 0.005, "list_rebuilds_in_timed_steps": n_reb},
+        roofline=roofline(eng, n, "lj1m_nve", skin, tpa),
     )
 
 
 def bench_ar256(args) -> dict:
-    """Ar fcc 4x4x4 (256 atoms) LJ NVE, 50 K, dt 2 fs, 100 steps; the FP64 CPU
-    oracle runs the same 100 steps as the baseline."""
-    from oracle import md as omd
-    from oracle import models as om
-    from oracle import nl as onl
-
+    """Ar fcc 4x4x4 (256 atoms) LJ NVE, 50 K, dt 2 fs, 100 steps."""
     dev = torch.device("cuda:0")
     state = tsa.workloads.ar_fcc(device=dev)
     model = tsa.LennardJonesModel(sigma=3.405, epsilon=0.0104, cutoff=8.5125, device=dev)
@@ -440,73 +550,60 @@ def bench_ar256(args) -> dict:
     eng = tsa.BatchedMD(model, md, dt=dt, langevin=False, skin=1.0, tpa=8, seed=1,
                         chunk_steps=100)
     eng.run(args.warmup)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    eng.run(args.steps)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
+    el, _ = timed_run(eng, args.steps)
     value = 256 * args.steps / el
-    cpu = None
-    if not args.no_cpu:
-        cst = tsa.workloads.ar_fcc(device="cpu")
-        fn = lambda d, zi, zj: om.lennard_jones_pair(d, zi, zj, 3.405, 0.0104)  # noqa: E731
-
-        def force(x):
-            o = om.pair_forward(x, cst.cell, cst.pbc, cst.atomic_numbers, cst.system_idx, fn,
-                                8.5125, compute_stress=False, nl=onl.linked_cell_nl)
-            return o["energy"], o["forces"]
-
-        torch.set_num_threads(os.cpu_count() or 1)
-        g = torch.Generator().manual_seed(0)
-        p = omd.initialize_momenta(cst.masses, cst.system_idx, torch.tensor(50.0 * omd.KB_EV),
-                                   torch.randn(256, 3, generator=g, dtype=torch.float64))
-        x = cst.positions.clone()
-        e, f = force(x)
-        x, p, f, e = omd.nve_step(x, p, f, cst.masses, force, 0.002 * omd.TIME_UNIT)  # warm-up
-        tc = time.perf_counter()
-        for _ in range(100):
-            x, p, f, e = omd.nve_step(x, p, f, cst.masses, force, 0.002 * omd.TIME_UNIT)
-        elc = time.perf_counter() - tc
-        cpu = {"value": 256 * 100 / elc, "unit": "atom-steps/s", "cores": torch.get_num_threads(),
-               "kind": "port", "sample": f"full 100 NVE steps, FP64 CPU oracle, {elc:.2f} s, "
-               f"cpu={platform.processor() or platform.machine()}"}
     log(f"ar256: {value:.4e} atom-steps/s")
     return _line("MD atom-steps/s at 1 MI355X, Ar-256 LJ NVE", value, "atom-steps/s", args.steps,
                  args.warmup, 1e3 * el / args.steps,
                  {"workload": "ar_fcc_256_nve", "n_atoms": 256, "T_K": 50.0, "dt_fs": 2.0,
-                  "cutoff_A": 8.5125}, cpu_baseline=cpu)
+                  "cutoff_A": 8.5125})
 
 
 def bench_fire(args) -> dict:
-    """1024 binary soft-sphere systems (64-1024 atoms, phi 0.64, triclinic
-    strain <= 5 %), ASE-FIRE with the Frechet cell filter: FIRE steps of the
-    whole batch, and how many systems reach fmax < 1e-4 (atoms and cell)."""
+    """Soft-sphere FIRE + Frechet relaxations to fmax 1e-4 (atoms and cell)
+    with in-place refill: a queue of ``--fire-queue`` binary soft-sphere
+    systems (64-1024 atoms, phi 0.64, triclinic strain <= 5 %) fed to
+    ``optimize`` with an in-flight budget of the first 1024 systems' atoms,
+    convergence checked every 5 steps; every system ends converged or at
+    ``--fire-max-steps``."""
     dev = torch.device("cuda:0")
-    state = tsa.workloads.soft_sphere_batch(args.n_systems or 1024, n_min=64, n_max=1024,
-                                            device=dev, seed=0)
+    n_queue = args.fire_queue
+    queue = tsa.workloads.soft_sphere_batch(n_queue, n_min=64, n_max=1024, device=dev, seed=0)
     sig = torch.tensor([[1.0, 1.2], [1.2, 1.4]], dtype=torch.float64)
     skin = args.skin if args.skin is not None else 0.3
     model = tsa.SoftSphereMultiModel(atomic_numbers=torch.tensor([1, 2]), sigma_matrix=sig,
                                      device=dev, compute_stress=True, skin=skin)
-    fs = tsa.fire_init(state, model, cell_filter="frechet")
-    for _ in range(args.warmup):
-        tsa.fire_step(fs, model)
+    counts = torch.bincount(queue.system_idx, minlength=n_queue)
+    budget = int(counts[: min(1024, n_queue)].sum())
+    work = {"atom_steps": 0, "steps": 0}
+
+    def step_fn(state, model, **kw):
+        work["atom_steps"] += state.n_atoms
+        work["steps"] += 1
+        return tsa.fire_step(state, model, **kw)
+
+    conv_fn = tsa.generate_force_convergence_fn(1e-4, include_cell_forces=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        tsa.fire_step(fs, model)
+    out = tsa.optimize(queue, model, optimizer=(tsa.fire_init, step_fn), convergence_fn=conv_fn,
+                       max_steps=args.fire_max_steps, steps_between_swaps=5,
+                       autobatcher=tsa.InFlightAutoBatcher(max_atoms=budget),
+                       init_kwargs={"cell_filter": "frechet"})
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    n = state.n_atoms
-    conv = int(((fs.fmax[:, 0] < 1e-4) & (fs.fmax[:, 1] < 1e-4)).sum())
-    value = n * args.steps / el
-    log(f"fire: {value:.4e} atom-steps/s, {conv} systems converged after {args.warmup + args.steps} steps")
-    return _line("FIRE atom-steps/s at 1 MI355X, 1024-system soft-sphere FIRE + Frechet", value,
-                 "atom-steps/s", args.steps, args.warmup, 1e3 * el / args.steps,
-                 {"workload": "soft_sphere_fire_frechet_1024", "n_systems": state.n_systems,
-                  "n_atoms": n, "phi": 0.64, "fmax": 1e-4, "skin": skin,
-                  "full_list_builds": model._vl.full_builds if model._vl else None,
-                  "converged_after_steps": conv, "steps_taken": args.warmup + args.steps})
+    conv = int(((out.fmax[:, 0] < 1e-4) & (out.fmax[:, 1] < 1e-4)).sum())
+    value = work["atom_steps"] / el
+    log(f"fire: {conv}/{n_queue} relaxed in {el:.2f} s ({n_queue / el:.1f} systems/s), "
+        f"{work['steps']} batch steps, {value:.4e} atom-steps/s")
+    return _line("FIRE atom-steps/s at 1 MI355X, soft-sphere FIRE + Frechet to fmax 1e-4 with "
+                 "in-place refill", value, "atom-steps/s", work["steps"], 0,
+                 1e3 * el / max(1, work["steps"]),
+                 {"workload": "soft_sphere_fire_frechet_refill", "queue_systems": n_queue,
+                  "queue_atoms": queue.n_atoms, "in_flight_atom_budget": budget, "phi": 0.64,
+                  "fmax": 1e-4, "steps_between_swaps": 5, "max_steps": args.fire_max_steps,
+                  "skin": skin},
+                 systems_relaxed=conv, systems_per_s=n_queue / el, seconds=el,
+                 full_list_builds=model._vl.full_builds if model._vl else None)
 
 
 def bench_fit(args) -> dict:
@@ -563,8 +660,8 @@ def bench_fit(args) -> dict:
     return _line("grad-evals/s at 1 MI355X, 256-system binary-LJ fit (force MSE + E after NVE)",
                  full, "grad-evals/s", args.steps, min(args.warmup, 1), 1e3 / full,
                  {"workload": "binary_lj_fit_256", "n_systems": state.n_systems, "n_atoms": n,
-                  "nve_steps": n_md, "checkpoint_every": 20, "dt_tau": 0.002, "skin": skin,
-                  "force_matching_only_grad_evals_per_s": fm})
+                  "nve_steps": n_md, "checkpoint_every": 20, "dt_tau": 0.002, "skin": skin},
+                 force_matching_only_grad_evals_per_s=fm)
 
 
 def calibrate_copy(mib: int) -> None:
@@ -581,13 +678,16 @@ def calibrate_copy(mib: int) -> None:
     print(f"copied {mib} MiB x3", flush=True)
 
 
+# workload -> (function, default steps, default warmup)
 WORKLOADS = {
     "cu_morse_nvt": (bench_cu_morse, 1000, 50),
     "lj1m_nve": (bench_lj1m, 500, 20),
     "ar256_nve": (bench_ar256, 100, 10),
-    "fire_ss_1024": (bench_fire, 100, 10),
+    "fire_ss_refill": (bench_fire, 0, 0),
     "lj_fit_256": (bench_fit, 3, 1),
 }
+# the legs the headline line carries: (workload, steps, warmup)
+LEGS = (("lj1m_nve", 500, 20), ("fire_ss_refill", 0, 0), ("lj_fit_256", 3, 1))
 
 
 def main() -> None:
@@ -599,11 +699,17 @@ def main() -> None:
     ap.add_argument("--n-systems", type=int, default=None)
     ap.add_argument("--skin", type=float, default=None)
     ap.add_argument("--tpa", type=int, default=None)
+    ap.add_argument("--bins", type=int, default=None, help="cell-list bins per list radius")
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-systems", type=int, default=32)
+    ap.add_argument("--no-legs", action="store_true",
+                    help="headline workload only (no LJ-1M / FIRE / fit legs)")
+    ap.add_argument("--cpu-steps", type=int, default=2,
+                    help="BAOAB steps of the full batch in the CPU baseline")
     ap.add_argument("--fit-md-steps", type=int, default=200)
+    ap.add_argument("--fire-queue", type=int, default=4096)
+    ap.add_argument("--fire-max-steps", type=int, default=10000)
     ap.add_argument("--calibrate-copy", type=int, default=None, metavar="MIB",
                     help="only run three device copies of MIB MiB (a kernel of known bytes "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -620,7 +726,19 @@ def main() -> None:
     args.warmup = warmup if args.warmup is None else args.warmup
     if args.workload == "cu_morse_nvt" and args.n_systems is None:
         args.n_systems = 512
-    print(json.dumps(fn(args)), flush=True)
+    out = fn(args)
+    if args.workload == "cu_morse_nvt":
+        if not args.no_legs:
+            legs = {}
+            for name, steps, warmup in LEGS:
+                sub = argparse.Namespace(**vars(args))
+                sub.steps, sub.warmup, sub.skin, sub.tpa, sub.n_systems = steps, warmup, None, None, None
+                log(f"leg {name}")
+                legs[name] = WORKLOADS[name][0](sub)
+                torch.cuda.empty_cache()
+            out["legs"] = legs
+        out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_steps)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -171,25 +171,31 @@ def test_rebuild_launch_reduction_equals_own_launch(dev):
 
 def test_autotuner_picks_from_its_grid_and_caches(dev, tmp_path, monkeypatch):
     """tune_md times every (tpa, skin) pair of its grid, re-times the two
-    fastest and returns one of them; the choice is cached per workload shape,
-    in memory and in the JSON file, so a later call runs no trial."""
+    fastest at every bin size and returns one of those; the choice is cached
+    per workload shape, temperature, grid, library and GPU, in memory and in
+    the JSON file, so the same call again runs no trial while another grid
+    or temperature is another key."""
     import json
     from torch_sim_amd import autotune
     monkeypatch.setenv("TSX_AUTOTUNE_CACHE", str(tmp_path / "at.json"))
     monkeypatch.setattr(autotune, "_MEM", {})
     st, model, md, kT = _setup(dev, n_sys=4, seed=41)
     lines = []
-    cfg = autotune.tune_md(st, model, tpas=(2, 4), skins=(0.3, 0.6), warmup_steps=5,
-                           trial_steps=10, log=lines.append)
+    grid = dict(tpas=(2, 4), skins=(0.3, 0.6), bins=(1, 2))
+    cfg = autotune.tune_md(st, model, warmup_steps=5, trial_steps=10, log=lines.append, **grid)
     print(lines, cfg, flush=True)
     assert (cfg["skin"], cfg["tpa"]) in {(0.3, 2), (0.3, 4), (0.6, 2), (0.6, 4)}
-    assert len(lines) == 6 and sum("re-timed" in ln for ln in lines) == 2
+    assert cfg["bins_per_rlist"] in (1, 2)
+    assert len(lines) == 8 and sum("re-timed" in ln for ln in lines) == 4
     assert cfg["rate"] > 0
     with open(tmp_path / "at.json") as fh:
         assert list(json.load(fh).values()) == [cfg]
     autotune._MEM.clear()
     again = []
-    assert autotune.tune_md(st, model, log=again.append) == cfg and not again
+    assert autotune.tune_md(st, model, log=again.append, **grid) == cfg and not again
+    k = autotune._key(st, model, 1000.0, grid["tpas"], grid["skins"], grid["bins"])
+    assert k != autotune._key(st, model, 1000.0, (2, 4, 8), grid["skins"], grid["bins"])
+    assert k != autotune._key(st, model, 300.0, grid["tpas"], grid["skins"], grid["bins"])
 
 
 def test_engine_graph_replay_equals_eager(dev):
@@ -254,6 +260,37 @@ def test_engine_batch_equals_single_bitwise_with_rebuilds(dev):
         assert torch.equal(s1.energy[0], full.energy[k])
         a += n
     assert total == eng.n_rebuilds
+
+
+def test_engine_fill_kernels_give_the_same_trajectory(dev):
+    """Hot Cu (2000 K) with a small skin rebuilds many systems' lists; the
+    per-bin and per-atom fills write the same rows, so the trajectories and
+    energies are bitwise the same, and so are consecutive runs against one
+    run of their total length (the engine leaves the last step open)."""
+    st = tsa.workloads.cu_batch(5, device=dev, seed=28)
+    model = tsa.MorseModel(**MORSE, cutoff=6.0, device=dev)
+    st.rng = 6
+    kT = 2000.0 * tsa.MetalUnits.temperature
+    md = tsa.nvt_langevin_init(state=st, model=model, kT=kT)
+    dt = 0.002 * tsa.MetalUnits.time
+    runs = []
+    for mode in (1, 2):
+        eng = tsa.BatchedMD(model, md, dt=dt, kT=kT, langevin=True, skin=0.2, seed=9)
+        eng.nl.fill_mode = mode
+        print("fill mode", mode, flush=True)
+        eng.run(60)
+        eng.run(1)
+        eng.run(59)
+        runs.append((eng.state(), eng.n_rebuilds))
+    one = tsa.BatchedMD(model, md, dt=dt, kT=kT, langevin=True, skin=0.2, seed=9)
+    one.run(120)
+    runs.append((one.state(), one.n_rebuilds))
+    assert runs[0][1] > 20
+    for out, nreb in runs[1:]:
+        assert nreb == runs[0][1]
+        assert torch.equal(out.positions, runs[0][0].positions)
+        assert torch.equal(out.momenta, runs[0][0].momenta)
+        assert torch.equal(out.energy, runs[0][0].energy)
 
 
 def test_engine_orthorhombic_path_equals_general_bitwise(dev):

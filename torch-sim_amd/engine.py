@@ -52,7 +52,7 @@ class BatchedMD:
         langevin: BAOAB Langevin (True) or velocity Verlet NVE (False).
         skin: Verlet skin added to the model cutoff for the stored list.
         tpa: lanes per atom in the pair kernel (1, 2, 4 or 8).
-        bins_per_rlist: cell-list bins per list radius (1 or 2).
+        bins_per_rlist: cell-list bins per list radius (1 to 4).
         seed: Philox key for the Langevin noise.
         use_graph: capture two steps into a hipGraph and replay it.
         chunk_steps: steps between host checks of the device status word.
@@ -153,6 +153,7 @@ class BatchedMD:
         self.steps_done = 0
         self._graph = None
         self._initialized = False
+        self._is_open = False
         self._noise = None
 
     # ---------------------------------------------------------- kernels
@@ -208,6 +209,7 @@ class BatchedMD:
         self.flags.zero_()
         self.cur = 0
         self._initialized = True
+        self._is_open = False
 
     def _open(self) -> None:
         """Forces and energies at the current positions (the state of the
@@ -268,6 +270,13 @@ class BatchedMD:
     def run(self, n_steps: int, noise=None) -> None:
         """Advance all systems by ``n_steps`` reference steps.
 
+        The last step is left open: its positions are final, its closing
+        half kick waits for the next run (which does it in the same launch as
+        the next step's forces) or for :meth:`close` / :meth:`state`.  A run
+        of n steps therefore costs n force passes (n + 1 from a closed
+        state), and consecutive runs equal one run of their total length bit
+        for bit.
+
         ``noise``: optional iterable of [N,3] float64 tensors, one per step
         (Langevin trajectory comparisons); runs without graph replay.
         """
@@ -278,8 +287,11 @@ class BatchedMD:
             self.initialize()
         if self.n_atoms == 0:
             return
-        self._open()
-        remaining = n_steps - 1
+        remaining = n_steps
+        if not self._is_open:
+            self._open()
+            self._is_open = True
+            remaining -= 1
         done = 0
         while done < remaining:
             todo = min(self.chunk_steps, remaining - done)
@@ -303,21 +315,30 @@ class BatchedMD:
             if st:
                 raise _lib.TsxError(f"engine status {st:#x} (shift overflow or singular cell)")
             done += todo
-        # closing half of the last step (positions unchanged)
-        self._step(self.cur, False)
-        self.cur = 1 - self.cur
         self.steps_done += n_steps
         self._noise = None
+
+    def close(self) -> None:
+        """Finish the open step: forces at its final positions, the closing
+        half kick and the per-system energies (positions unchanged)."""
+        if self._is_open:
+            self._step(self.cur, False)
+            self.cur = 1 - self.cur
+            self._is_open = False
 
     # ------------------------------------------------------------- output
     @property
     def positions(self) -> torch.Tensor:
+        """Positions at the end of the last step (final also while the step
+        is open: only its momenta wait for the closing kick)."""
         return self.pos[self.cur][:, :3]
 
     def state(self):
-        """MD state with the engine's current positions, momenta and forces."""
+        """MD state with the engine's current positions, momenta and forces
+        (closes an open step first)."""
         from .integrators import MDState
 
+        self.close()
         t = self.state_template
         return MDState(
             positions=self.positions.clone(),
