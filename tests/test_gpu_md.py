@@ -293,6 +293,27 @@ def test_engine_fill_kernels_give_the_same_trajectory(dev):
         assert torch.equal(out.energy, runs[0][0].energy)
 
 
+def test_engine_streamed_list_words_change_nothing(dev):
+    """Non-temporal loads of the list words (the engine's choice for systems
+    above 16,384 atoms) give bitwise the trajectory of plain loads."""
+    st = tsa.workloads.cu_batch(3, device=dev, seed=29)
+    model = tsa.MorseModel(**MORSE, cutoff=6.0, device=dev)
+    st.rng = 7
+    kT = 1500.0 * tsa.MetalUnits.temperature
+    md = tsa.nvt_langevin_init(state=st, model=model, kT=kT)
+    dt = 0.002 * tsa.MetalUnits.time
+    runs = []
+    for streamed in (False, True):
+        eng = tsa.BatchedMD(model, md, dt=dt, kT=kT, langevin=True, skin=0.3, seed=4)
+        assert not eng.stream_words  # small systems: plain loads by default
+        eng.stream_words = streamed
+        eng.run(50)
+        runs.append(eng.state())
+    assert torch.equal(runs[0].positions, runs[1].positions)
+    assert torch.equal(runs[0].momenta, runs[1].momenta)
+    assert torch.equal(runs[0].energy, runs[1].energy)
+
+
 def test_engine_orthorhombic_path_equals_general_bitwise(dev):
     """Cubic cells take the fused kernel's orthorhombic stencil shift (one
     product per component); forcing the general three-term form must give the

@@ -63,7 +63,7 @@ _SIGS = {
         [
             I, P, I, P, P, P, P, P, P, P, P, P, L, I, I, L, D,  # pair part
             P, P, P, P, P, D, I, P, P, P, U, P, I, I, D, P,  # step part
-            P, P, P, P, I, P,  # frame: rel_in, rel_out, anchor, ustep; orthorhombic; stream
+            P, P, P, P, I, I, P,  # rel_in, rel_out, anchor, ustep; orthorhombic; streamed words
         ],
     ),
     "tsx_md_reduce": (I, [P, P, I, I, P, P, I, P, P, P]),

@@ -114,6 +114,7 @@ struct FusedArgs {
   const double* anchor; // [N,4] bin anchors A
   const double* ustep;  // [S,12] a_k / nb_k (rows), sum_k |a_k / nb_k|
   int ortho;            // every cell orthorhombic (ustep off-diagonals zero)
+  int stream_words;     // load the list words non-temporally
 };
 
 // The FP64 side of the cutoff decision for entries within the rounding
@@ -176,7 +177,9 @@ __device__ __forceinline__ void stencil_disp(float fx, float fy, float fz, const
   }
 }
 
-template <int KIND, int TPA, bool VIRIAL, bool MULTI, bool ORTHO>
+// NTW: load the list words non-temporally (a compile-time choice: as a
+// runtime branch the compiler merges the two loads and drops the hint)
+template <int KIND, int TPA, bool VIRIAL, bool MULTI, bool ORTHO, bool NTW>
 __device__ __forceinline__ RowAcc accumulate_rel(const PairP* sp, int ns, int ti,
                                                  const float4* __restrict__ rel_s, float xri,
                                                  float yri, float zri, const float* uf,
@@ -196,7 +199,9 @@ __device__ __forceinline__ RowAcc accumulate_rel(const PairP* sp, int ns, int ti
     uint32_t w[UNR_REL];
 #pragma unroll
     for (int u = 0; u < UNR_REL; ++u)
-      w[u] = (k0 + u < iters) ? wp[(int64_t)(k0 + u) * stride] : 0u;
+      w[u] = (k0 + u < iters) ? (NTW ? __builtin_nontemporal_load(wp + (int64_t)(k0 + u) * stride)
+                                     : wp[(int64_t)(k0 + u) * stride])
+                              : 0u;
     float4 q[UNR_REL];
 #pragma unroll
     for (int u = 0; u < UNR_REL; ++u) q[u] = rel_s[w[u] & jmask];
@@ -262,7 +267,7 @@ __device__ __forceinline__ RowAcc accumulate_rel(const PairP* sp, int ns, int ti
 // gains more from the fourth wave's loads in flight than it loses to the
 // tighter budget (they fit it without spills).  The virial and soft-sphere
 // variants keep the default budget: capped, they would spill.
-template <int KIND, int TPA, bool LANGEVIN, bool VIRIAL, bool MULTI, bool ORTHO>
+template <int KIND, int TPA, bool LANGEVIN, bool VIRIAL, bool MULTI, bool ORTHO, bool NTW = false>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(VIRIAL || KIND == TSX_SOFT_SPHERE ? 1 : 4)))
 void k_md_fused(FusedArgs A) {
@@ -305,7 +310,7 @@ void k_md_fused(FusedArgs A) {
   ex.rc2 = A.rc * A.rc;
   const float4* rel_s = reinterpret_cast<const float4*>(A.rel_in) + base;
   const int cnt = valid ? A.count[tt] : 0;
-  const RowAcc a = accumulate_rel<KIND, TPA, VIRIAL, MULTI, ORTHO>(
+  const RowAcc a = accumulate_rel<KIND, TPA, VIRIAL, MULTI, ORTHO, NTW>(
       sp, A.ns, ti, rel_s, (float)ex.xi, (float)ex.yi, (float)ex.zi, uf, A.nbr, tt, cnt, lane,
       A.jbits, A.ld, lo2, hi2, ex);
   // Philox normals of (seed, step, atom) for the O step: with two or more
@@ -411,6 +416,10 @@ static void launch_fused_v(bool vir, hipStream_t st, int nblk, const FusedArgs& 
     k_md_fused<KIND, TPA, LANGEVIN, false, true, true><<<nblk, 256, 0, st>>>(a);
   else if (multi)
     k_md_fused<KIND, TPA, LANGEVIN, false, true, false><<<nblk, 256, 0, st>>>(a);
+  else if (a.ortho && a.stream_words)  // one large box: words streamed past L2
+    k_md_fused<KIND, TPA, LANGEVIN, false, false, true, true><<<nblk, 256, 0, st>>>(a);
+  else if (a.stream_words)
+    k_md_fused<KIND, TPA, LANGEVIN, false, false, false, true><<<nblk, 256, 0, st>>>(a);
   else if (a.ortho)
     k_md_fused<KIND, TPA, LANGEVIN, false, false, true><<<nblk, 256, 0, st>>>(a);
   else
@@ -489,6 +498,7 @@ int tsx_md_fused_step(int32_t kind, const double* params, int32_t n_species,
                       uint64_t seed, const int64_t* step, int32_t end_kick, int32_t do_next,
                       double half_skin, int32_t* flag_set, const float* rel_in, float* rel_out,
                       const double* anchor, const double* ustep, int32_t orthorhombic,
+                      int32_t stream_words,
                       void* stream) {
   (void)cell_col;  // the frame (anchor, ustep) carries the geometry
   TSX_REQUIRE(kind >= TSX_LJ && kind <= TSX_SOFT_SPHERE, "unknown pair kind %d", kind);
@@ -535,6 +545,7 @@ int tsx_md_fused_step(int32_t kind, const double* params, int32_t n_species,
   a.anchor = anchor;
   a.ustep = ustep;
   a.ortho = orthorhombic != 0;
+  a.stream_words = stream_words != 0;
   hipStream_t st = (hipStream_t)stream;
   const int nblk = blocks_for(n_atoms * tpa, 256);
   const bool vir = w_atom != nullptr;

@@ -143,6 +143,9 @@ class BatchedMD:
         # runs inside the next step's rebuild launch; the last step of a run
         # and the other layouts keep the reduction launch of their own.
         self._merge_reduce = self.nl.one_block_build and 0 < self.sysidx.max_atoms <= 2048
+        # a system this large reuses a neighbour's position only after many
+        # megabytes of list words: stream the words past L2
+        self.stream_words = self.sysidx.max_atoms > 16384
         self._frames_red = [
             _lib.NLFrame(self.anchor.data_ptr(), self.rel[q].data_ptr(), self.ustep.data_ptr(),
                          self.e_atom.data_ptr(), self.energy.data_ptr(),
@@ -173,7 +176,7 @@ class BatchedMD:
             int(self.langevin), ptr(self.kT), ptr(self.gamma), ptr(noise), self.seed,
             ptr(self.step_ctr), int(end_kick), int(do_next), 0.5 * self.skin,
             ptr(self._gate(1 - q)), ptr(self.rel[q]), ptr(self.rel[1 - q]), ptr(self.anchor),
-            ptr(self.ustep), int(self.orthorhombic), stream(),
+            ptr(self.ustep), int(self.orthorhombic), int(self.stream_words), stream(),
         )
 
     def _reduce(self, q: int) -> None:
