@@ -575,34 +575,29 @@ def bench_fire(args) -> dict:
                                      device=dev, compute_stress=True, skin=skin)
     counts = torch.bincount(queue.system_idx, minlength=n_queue)
     budget = int(counts[: min(1024, n_queue)].sum())
-    work = {"atom_steps": 0, "steps": 0}
-
-    def step_fn(state, model, **kw):
-        work["atom_steps"] += state.n_atoms
-        work["steps"] += 1
-        return tsa.fire_step(state, model, **kw)
-
+    work: dict = {}
     conv_fn = tsa.generate_force_convergence_fn(1e-4, include_cell_forces=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    out = tsa.optimize(queue, model, optimizer=(tsa.fire_init, step_fn), convergence_fn=conv_fn,
+    out = tsa.optimize(queue, model, optimizer=tsa.Optimizer.fire, convergence_fn=conv_fn,
                        max_steps=args.fire_max_steps, steps_between_swaps=5,
                        autobatcher=tsa.InFlightAutoBatcher(max_atoms=budget),
-                       init_kwargs={"cell_filter": "frechet"})
+                       init_kwargs={"cell_filter": "frechet"}, stats=work)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     conv = int(((out.fmax[:, 0] < 1e-4) & (out.fmax[:, 1] < 1e-4)).sum())
     value = work["atom_steps"] / el
     log(f"fire: {conv}/{n_queue} relaxed in {el:.2f} s ({n_queue / el:.1f} systems/s), "
-        f"{work['steps']} batch steps, {value:.4e} atom-steps/s")
+        f"{work['batch_steps']} batch steps, {work['refills']} refills, {value:.4e} atom-steps/s")
     return _line("FIRE atom-steps/s at 1 MI355X, soft-sphere FIRE + Frechet to fmax 1e-4 with "
-                 "in-place refill", value, "atom-steps/s", work["steps"], 0,
-                 1e3 * el / max(1, work["steps"]),
+                 "in-place refill", value, "atom-steps/s", work["batch_steps"], 0,
+                 1e3 * el / max(1, work["batch_steps"]),
                  {"workload": "soft_sphere_fire_frechet_refill", "queue_systems": n_queue,
                   "queue_atoms": queue.n_atoms, "in_flight_atom_budget": budget, "phi": 0.64,
                   "fmax": 1e-4, "steps_between_swaps": 5, "max_steps": args.fire_max_steps,
                   "skin": skin},
                  systems_relaxed=conv, systems_per_s=n_queue / el, seconds=el,
+                 refills=work["refills"], swap_checks=work["swaps"],
                  full_list_builds=model._vl.full_builds if model._vl else None)
 
 

@@ -191,3 +191,60 @@ def test_optimize_energy_convergence_default(dev):
     out = tsa.optimize(st, model, optimizer=tsa.Optimizer.fire, max_steps=400)
     assert out.n_systems == 2
     assert bool((model(out)["energy"] <= model(st)["energy"]).all())
+
+
+def test_optimize_resident_equals_host_path_before_any_swap(dev):
+    """With no system converging before max_steps, the device-resident path
+    (batch gathered from the initialised pool, list rows relocated) and the
+    host path (batch selected and initialised on its own, list built for
+    it) step the same batch: bitwise the same result."""
+    st, _, _ = _ss(dev, n_sys=5, n_min=32, n_max=96, seed=14)
+    never = lambda state, last_energy: torch.zeros(state.n_systems, dtype=torch.bool,  # noqa: E731
+                                                   device=state.device)
+    outs = []
+    for resident in (True, False):
+        model = tsa.SoftSphereMultiModel(atomic_numbers=torch.tensor([1, 2]), sigma_matrix=SIG,
+                                         device=dev, compute_stress=True, skin=0.3)
+        opt = (tsa.fire_init, tsa.fire_step) if resident else (
+            lambda **kw: tsa.fire_init(**kw), lambda **kw: tsa.fire_step(**kw))
+        stats = {}
+        print("resident", resident, flush=True)
+        with pytest.warns(UserWarning, match="maximum number of steps"):
+            outs.append(tsa.optimize(st, model, optimizer=opt, convergence_fn=never, max_steps=40,
+                                     init_kwargs=dict(cell_filter="frechet"), stats=stats))
+        assert stats["batch_steps"] == 40 and stats["refills"] == 0
+    for name in ("positions", "cell", "velocities", "forces", "energy", "dt", "alpha", "n_pos"):
+        assert torch.equal(getattr(outs[0], name), getattr(outs[1], name)), name
+
+
+def test_optimize_resident_refill_with_skin_converges_in_order(dev):
+    """A tight atom budget forces many refills with the list rows relocated
+    (skin 0.3): every system is returned in input order, relaxed below the
+    tolerance by an independent model call, at or below its initial energy,
+    and two runs give the same bits."""
+    st, _, _ = _ss(dev, n_sys=10, n_min=32, n_max=128, seed=15)
+    conv = tsa.generate_force_convergence_fn(1e-3, include_cell_forces=True)
+    runs = []
+    for _ in range(2):
+        model = tsa.SoftSphereMultiModel(atomic_numbers=torch.tensor([1, 2]), sigma_matrix=SIG,
+                                         device=dev, compute_stress=True, skin=0.3)
+        stats = {}
+        out = tsa.optimize(st, model, optimizer=tsa.Optimizer.fire, convergence_fn=conv,
+                           max_steps=5000, init_kwargs=dict(cell_filter="frechet"),
+                           autobatcher=tsa.InFlightAutoBatcher(max_atoms=260), stats=stats)
+        print(stats, flush=True)
+        assert stats["refills"] >= 4 and stats["atom_steps"] > 0
+        runs.append(out)
+    out = runs[0]
+    assert out.n_systems == st.n_systems
+    assert torch.equal(out.system_idx, st.system_idx)
+    assert torch.equal(out.atomic_numbers, st.atomic_numbers)
+    fresh = tsa.SoftSphereMultiModel(atomic_numbers=torch.tensor([1, 2]), sigma_matrix=SIG,
+                                     device=dev, compute_stress=True)
+    res = fresh(out)
+    f_atoms = torch.zeros(st.n_systems, dtype=F64, device=dev).scatter_reduce(
+        0, out.system_idx, res["forces"].norm(dim=1), reduce="amax", include_self=False)
+    assert bool((f_atoms < 1.5e-3).all()), f_atoms
+    assert bool((res["energy"] <= fresh(st)["energy"] + 1e-12).all())
+    for name in ("positions", "cell", "energy"):
+        assert torch.equal(getattr(runs[0], name), getattr(runs[1], name)), name

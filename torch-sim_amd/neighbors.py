@@ -482,6 +482,74 @@ class VerletList:
         return False
 
 
+def segment_index(src_starts: torch.Tensor, counts: torch.Tensor, total: int) -> torch.Tensor:
+    """The ranges [src_starts[k], src_starts[k] + counts[k]) concatenated
+    (device int64; ``total`` = sum of counts, given by the caller so that
+    nothing synchronises)."""
+    dst_starts = torch.cumsum(counts, 0) - counts
+    return torch.arange(total, device=counts.device) + torch.repeat_interleave(
+        src_starts - dst_starts, counts, output_size=total)
+
+
+def assemble_verlet(dst: VerletList, parts, sysidx: SystemIndex) -> None:
+    """Make ``dst`` the Verlet list of the layout ``sysidx`` from rows of
+    other Verlet lists, without a build.
+
+    ``parts`` is a sequence of ``(src, src_sysidx, sys_ids, n_atoms)``: the
+    systems ``sys_ids`` (device int64, in order) of a list ``src`` valid for
+    the layout ``src_sysidx``, holding ``n_atoms`` atoms together; the new
+    layout is the concatenation of the parts' systems.  A system's rows are
+    the contiguous block of its atom range and its list words index atoms
+    within the system, so the rows move with block gathers; only the row
+    order entries shift to the new atom offsets.  The displacement
+    references (positions and cells at the last build) move with their
+    systems, so every system keeps its rebuild schedule.  All sources need
+    the same word format (jbits, lanes per atom, list radius)."""
+    srcs = [p[0] for p in parts]
+    nl0 = srcs[0].nl
+    for v in srcs:
+        if (v.nl.jbits, v.nl.tpa, v.nl.r_list) != (nl0.jbits, nl0.tpa, nl0.r_list):
+            raise ValueError("assembled Verlet lists need one word format")
+    n, s = sysidx.n_atoms, sysidx.n_systems
+    dev = sysidx.atom_sys.device
+    kmax = max(v.nl.kmax for v in srcs)
+    bin_cap = max(v.nl.bin_cap for v in srcs)
+    nl = dst.nl
+    nl.n_atoms = -1  # reallocate for the new layout
+    nl.ensure(n, s, kmax, bin_cap, nl0.jbits, dev)
+    tpa = nl.tpa
+    out = nl.nbr.view(nl.kmax // tpa, nl.ld, tpa)
+    dst.xref = torch.zeros((max(1, n), 4), dtype=torch.float64, device=dev)
+    dst.cell_build = torch.empty((s, 3, 3), dtype=torch.float64, device=dev)
+    dst.flags = torch.zeros(s, dtype=torch.int32, device=dev)
+    nbytes = int(_lib.lib().tsx_nl_check_ws_bytes(s))
+    dst.ws = torch.empty((nbytes + 7) // 8, dtype=torch.float64, device=dev)
+    a0 = s0 = 0
+    for src, ssi, ids, n_part in parts:
+        k = ids.numel()
+        if k == 0:
+            continue
+        cnt = (ssi.sys_start[ids + 1] - ssi.sys_start[ids]).to(torch.int64)
+        old_start = ssi.sys_start[ids].to(torch.int64)
+        rows = segment_index(old_start, cnt, n_part)
+        new_start = sysidx.sys_start[s0:s0 + k].to(torch.int64)
+        shift = torch.repeat_interleave((new_start - old_start).to(torch.int32), cnt,
+                                        output_size=n_part)
+        sn = src.nl
+        kg = sn.kmax // tpa
+        src_nbr = sn.nbr.view(kg, sn.ld, tpa)
+        out[:kg, a0:a0 + n_part, :] = src_nbr.index_select(1, rows)
+        nl.count[a0:a0 + n_part] = sn.count.index_select(0, rows)
+        nl.order[a0:a0 + n_part] = sn.order.index_select(0, rows) + shift
+        dst.xref[a0:a0 + n_part] = src.xref.index_select(0, rows)
+        dst.cell_build[s0:s0 + k] = src.cell_build.index_select(0, ids)
+        a0 += n_part
+        s0 += k
+    nl.status.zero_()
+    dst._key = sysidx.serial
+    dst._gen += 1
+
+
 def _normalize(cell: torch.Tensor, pbc: torch.Tensor, n_systems: int):
     """Cell -> [S,3,3], pbc -> [S,3] (torch_sim/neighbors/utils.py:7-34)."""
     if cell.ndim == 2:

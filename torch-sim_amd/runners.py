@@ -23,10 +23,15 @@ from typing import Any, Callable
 
 import torch
 
+from dataclasses import fields as dc_fields
+
+import numpy as np
+
 from .engine import BatchedMD, estimate_skin
 from .integrators import INTEGRATOR_KWARG_UNITS, INTEGRATOR_REGISTRY, Integrator, MDState
 from .models import PairPotentialModel
-from .optimizers import OPTIM_REGISTRY, Optimizer
+from .neighbors import SystemIndex, TensorKey, VerletList, assemble_verlet, segment_index
+from .optimizers import OPTIM_REGISTRY, Optimizer, fire_init, fire_step
 from .quantities import system_wise_max_force
 from .state import SimState, concatenate_states, initialize_state, select_systems
 from .units import MetalUnits
@@ -170,6 +175,183 @@ class InFlightAutoBatcher:
         self.max_iterations = max_iterations
 
 
+def _gather_systems(parts, cls, template, counts_new: np.ndarray):
+    """A state of class ``cls`` holding, in order, the systems of several
+    states: ``parts`` is a sequence of (state, sys_ids (device int64),
+    atom_rows (device int64), n_atoms).  Per-atom and per-system tensors are
+    device gathers; ``system_idx`` and the system offsets come from the
+    host-side ``counts_new`` (no synchronisation)."""
+    dev = template.device
+    kw: dict[str, Any] = {}
+    n_new = int(counts_new.sum())
+    cnt = torch.as_tensor(counts_new, device=dev, dtype=torch.int64)
+    for f in dc_fields(cls):
+        name = f.name
+        v0 = getattr(template, name)
+        if name == "system_idx":
+            kw[name] = torch.repeat_interleave(torch.arange(len(counts_new), device=dev), cnt,
+                                               output_size=n_new)
+        elif name == "_rng":
+            kw[name] = None
+        elif isinstance(v0, torch.Tensor) and name in cls._atom_attributes:
+            kw[name] = torch.cat([p[0].__dict__[name].index_select(0, p[2]) for p in parts
+                                  if p[3]]).contiguous()
+        elif isinstance(v0, torch.Tensor) and name in cls._system_attributes:
+            kw[name] = torch.cat([p[0].__dict__[name].index_select(0, p[1]) for p in parts
+                                  if p[1].numel()]).contiguous()
+        else:
+            kw[name] = v0
+    return cls(**kw)
+
+
+def _scatter_systems(dst, src, dst_ids, dst_rows, src_ids, src_rows) -> None:
+    """dst's systems ``dst_ids`` (atoms ``dst_rows``) <- src's systems
+    ``src_ids`` (atoms ``src_rows``), every per-atom and per-system tensor."""
+    cls = type(dst)
+    for f in dc_fields(cls):
+        name = f.name
+        v = dst.__dict__.get(name)
+        if not isinstance(v, torch.Tensor) or name == "system_idx":
+            continue
+        if name in cls._atom_attributes:
+            v.index_copy_(0, dst_rows, src.__dict__[name].index_select(0, src_rows))
+        elif name in cls._system_attributes:
+            v.index_copy_(0, dst_ids, src.__dict__[name].index_select(0, src_ids))
+
+
+def _resident_ok(model, init_fn, step_fn) -> bool:
+    return isinstance(model, PairPotentialModel) and init_fn is fire_init and step_fn is fire_step
+
+
+def _optimize_resident(state0, model, *, convergence_fn, max_steps, steps_between_swaps,
+                       budget, init_kwargs, optimizer_kwargs, stats):
+    """In-flight FIRE with a device-resident refill (autobatching.py:1040-1152,
+    runners.py:567-753).
+
+    Every structure is FIRE-initialised once, as the reference does before it
+    starts swapping (runners.py:650-667): the resulting pool state, with its
+    Verlet list, holds each system at its input position.  The batch in
+    flight is gathered from the pool; every ``steps_between_swaps`` steps one
+    host read of the convergence mask decides the swap: converged systems
+    are written back into the pool, and the next batch is gathered on the
+    device -- the kept systems from the current batch, then as many queued
+    systems as the atom budget allows, in queue order.  Their neighbor-list
+    rows move with them (``assemble_verlet``), so a swap rebuilds no list,
+    and nothing is concatenated on the host.  The pool ends up as the result,
+    in input order."""
+    dev = state0.device
+    pool = fire_init(state0, model, **init_kwargs)
+    n_total = pool.n_systems
+    psi = model._system_index(pool)
+    counts = psi.counts.astype(np.int64)
+    pstart = np.concatenate([[0], np.cumsum(counts)])
+    pool_vl = model._vl
+    pool_species = model.species_of(pool.atomic_numbers)
+    if pool_vl is not None:  # the pool keeps its list; the batch gets its own
+        model._vl = VerletList(model.cutoff, model.skin, tpa=model.tpa)
+    steps = np.ones(n_total, dtype=np.int64)  # the reference counts from 1 (runners.py:160-162)
+    queue = list(range(n_total))
+
+    def take(n_free_atoms: int, *, at_least_one: bool) -> list[int]:
+        out, used = [], 0
+        while queue:
+            c = int(counts[queue[0]])
+            if used + c > n_free_atoms and (out or not at_least_one):
+                break
+            used += c
+            out.append(queue.pop(0))
+        return out
+
+    def rows_of(start: np.ndarray, cnt: np.ndarray) -> torch.Tensor:
+        return segment_index(torch.as_tensor(start, device=dev, dtype=torch.int64),
+                             torch.as_tensor(cnt, device=dev, dtype=torch.int64),
+                             int(cnt.sum()))
+
+    def assemble(flight, fsi, keep_local: list[int], new: list[int], current: list[int]):
+        keep_pool = [current[k] for k in keep_local]
+        counts_new = counts[keep_pool + new]
+        k_ids = torch.as_tensor(keep_local, device=dev, dtype=torch.int64)
+        n_ids = torch.as_tensor(new, device=dev, dtype=torch.int64)
+        parts = []
+        if keep_local:
+            fstart = np.concatenate([[0], np.cumsum(counts[current])])
+            kl = np.asarray(keep_local)
+            parts.append((flight, k_ids, rows_of(fstart[kl], counts[current][kl]),
+                          int(counts[keep_pool].sum())))
+        if new:
+            na = np.asarray(new)
+            parts.append((pool, n_ids, rows_of(pstart[na], counts[na]), int(counts[na].sum())))
+        nxt = _gather_systems(parts, type(pool), pool, counts_new)
+        si = SystemIndex(nxt.system_idx, len(counts_new))
+        si._counts = counts_new.copy()
+        object.__setattr__(nxt, "_tsx_sysidx", si)
+        model._sysidx = si
+        if pool_species is not None:
+            sp = []
+            if keep_local:
+                sp.append(model._species.index_select(0, parts[0][2]))
+            if new:
+                sp.append(pool_species.index_select(0, parts[-1][2]))
+            model._species_key, model._species = TensorKey(nxt.atomic_numbers), torch.cat(sp)
+        if pool_vl is not None:
+            vparts = []
+            if keep_local:
+                vparts.append((model._vl, fsi, k_ids, parts[0][3]))
+            if new:
+                vparts.append((pool_vl, psi, n_ids, parts[-1][3]))
+            dst = VerletList(model.cutoff, model.skin, tpa=model.tpa)
+            assemble_verlet(dst, vparts, si)
+            model._vl = dst
+        if keep_local:  # kept systems carry velocities: not a first step
+            nxt._fire_velocities_seen = (nxt.velocities, nxt.velocities._version)
+        return nxt, si
+
+    current = take(budget, at_least_one=True)
+    state, fsi = assemble(None, None, [], current, [])
+    n_swaps = n_refills = 0
+    while True:
+        exceeded = np.zeros(len(current), dtype=bool)
+        last_energy = state.energy
+        for _ in range(steps_between_swaps):
+            last_energy = state.energy
+            state = fire_step(state, model, **optimizer_kwargs)
+            stats["atom_steps"] += state.n_atoms
+            stats["batch_steps"] += 1
+            steps[current] += 1
+            exceeded = steps[current] > max_steps
+            if bool((steps > max_steps).all()):
+                warnings.warn(f"All systems have reached the maximum number of steps: {max_steps}.",
+                              stacklevel=3)
+                break
+        conv = convergence_fn(state, last_energy).to("cpu").numpy() | exceeded
+        n_swaps += 1
+        if not conv.any():
+            continue
+        conv_local = np.nonzero(conv)[0]
+        keep_local = np.nonzero(~conv)[0].tolist()
+        # converged systems back into the pool (the result, in input order)
+        cur = np.asarray(current)
+        fstart = np.concatenate([[0], np.cumsum(counts[cur])])
+        done = cur[conv_local]
+        _scatter_systems(pool, state, torch.as_tensor(done, device=dev),
+                         rows_of(pstart[done], counts[done]),
+                         torch.as_tensor(conv_local, device=dev),
+                         rows_of(fstart[conv_local], counts[done]))
+        kept_atoms = int(counts[cur[keep_local]].sum()) if keep_local else 0
+        new = take(budget - kept_atoms, at_least_one=not keep_local) if queue else []
+        if not keep_local and not new:
+            break
+        n_refills += 1
+        state, fsi = assemble(state, fsi, keep_local, new, current)
+        current = [current[k] for k in keep_local] + new
+    stats["swaps"] += n_swaps
+    stats["refills"] += n_refills
+    # the pool keeps the model's list; later calls on other states rebuild
+    model._sysidx = None
+    model._species_key = None
+    return pool
+
+
 def optimize(
     system,
     model,
@@ -182,13 +364,18 @@ def optimize(
     autobatcher: InFlightAutoBatcher | bool = False,
     pbar: bool = False,  # noqa: ARG001
     init_kwargs: dict[str, Any] | None = None,
+    stats: dict | None = None,
     **optimizer_kwargs: Any,
 ):
     """Relax every system to convergence (runners.py:567-753).
 
     The returned state holds all systems in input order; systems that hit
     ``max_steps`` are returned as they are (check ``convergence_fn`` on the
-    result)."""
+    result).  FIRE with one of this package's pair models runs the
+    device-resident refill of ``_optimize_resident``; other optimizer pairs
+    or models run the same schedule with host-side selection and
+    concatenation.  ``stats`` (optional dict) receives the atom-steps, batch
+    steps, swap checks and refills of the run."""
     if trajectory_reporter is not None:
         raise NotImplementedError("trajectory reporting is outside this package")
     if convergence_fn is None:
@@ -210,9 +397,18 @@ def optimize(
         budget = autobatcher.max_atoms
     else:
         budget = sum(counts)  # everything in flight (one GPU holds these batches)
+    if stats is None:
+        stats = {}
+    for k in ("atom_steps", "batch_steps", "swaps", "refills"):
+        stats.setdefault(k, 0)
+    if _resident_ok(model, init_fn, step_fn):
+        return _optimize_resident(state0, model, convergence_fn=convergence_fn,
+                                  max_steps=max_steps, steps_between_swaps=steps_between_swaps,
+                                  budget=budget, init_kwargs=init_kwargs,
+                                  optimizer_kwargs=optimizer_kwargs, stats=stats)
     queue = list(range(n_total))
     done: dict[int, SimState] = {}
-    steps = torch.zeros(n_total, dtype=torch.int64)
+    steps = torch.ones(n_total, dtype=torch.int64)  # counted from 1 (runners.py:160-162)
 
     def take(n_free_atoms: int, *, at_least_one: bool) -> list[int]:
         out, used = [], 0
@@ -232,6 +428,8 @@ def optimize(
         for _ in range(steps_between_swaps):
             last_energy = state.energy
             state = step_fn(state=state, model=model, **optimizer_kwargs)
+            stats["atom_steps"] += state.n_atoms
+            stats["batch_steps"] += 1
             steps[current] += 1
             exceeded = steps > max_steps
             if bool(exceeded.all()):
@@ -240,6 +438,7 @@ def optimize(
                 break
         conv = convergence_fn(state, last_energy).to("cpu")
         conv = conv | exceeded[current]
+        stats["swaps"] += 1
         if not bool(conv.any()):
             continue
         conv_local = torch.nonzero(conv).flatten().tolist()
@@ -258,6 +457,7 @@ def optimize(
             parts.append(init_fn(state=select_systems(state0, new), model=model, **init_kwargs))
         state = parts[0] if len(parts) == 1 else concatenate_states(parts)
         current = [current[loc] for loc in keep_local] + new
+        stats["refills"] += 1
     # input order
     groups: dict[int, tuple] = {}
     for sys_id, (st, k) in done.items():
