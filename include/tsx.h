@@ -441,6 +441,44 @@ int tsx_pair_grad(int32_t kind, const double* params, int32_t n_species, const i
                   const double* w, const double* ebar, double* dpos, double* dcell,
                   double* dparams, int32_t accumulate, void* ws, void* stream);
 
+/* ------------------------------------------- isotropic NPT Langevin (GJF) --- */
+
+/* Workspace bytes for the NPT step: per-system strain force, cell noise,
+ * old strain and scale (4 doubles) and the per-atom noise beta (3 doubles),
+ * carried from _pre to _post. */
+int64_t tsx_npt_ws_bytes(int64_t n_atoms, int32_t n_systems);
+
+/* First half of one isotropic NPT Langevin step (replaces
+ * torch_sim/integrators/npt.py:987-1070, the cell and particle position
+ * updates of npt_langevin_isotropic_step; state npt.py:653-701): per system,
+ * cell_masses = (n + 1) kT b_tau^2, the strain force 3 V (P - p_ext) with
+ * P = -tr(stress) / 3 + n kT / V and V = det(ref_cell) exp(3 eps), the GJF
+ * strain update eps -> eps' and cell_col = exp(eps') ref_cell; per atom,
+ * beta = sqrt(2 alpha kT dt) noise and the GJF position update scaled by
+ * exp(eps' - eps).  cell_noise [S] and noise [N,3] are standard normals.
+ * All arrays FP64, per-system ones [S] (cells [S,3,3] column vectors). */
+int tsx_npt_langevin_iso_pre(int32_t n_systems, const int32_t* sys_start,
+                             const int32_t* atom_sys, int64_t n_atoms, double* pos,
+                             const double* mom, const double* forces, const double* masses,
+                             const double* kT, const double* p_ext, const double* alpha,
+                             const double* cell_alpha, const double* b_tau, double dt,
+                             const double* stress, const double* ref_cell, double* eps,
+                             const double* eps_dot, double* cell_masses, double* cell_col,
+                             const double* cell_noise, const double* noise, void* ws,
+                             void* stream);
+
+/* Second half, after the model call at the new positions and cell
+ * (npt.py:1072-1107): the strain velocity from the old and new strain
+ * forces, and the GJF momentum update from f_old, f_new and the beta of
+ * tsx_npt_langevin_iso_pre (same ws). */
+int tsx_npt_langevin_iso_post(int32_t n_systems, const int32_t* sys_start,
+                              const int32_t* atom_sys, int64_t n_atoms, double* mom,
+                              const double* f_old, const double* f_new, const double* masses,
+                              const double* kT, const double* p_ext, const double* alpha,
+                              const double* cell_alpha, double dt, const double* stress,
+                              const double* ref_cell, const double* eps, double* eps_dot,
+                              const double* cell_masses, const void* ws, void* stream);
+
 /* One reverse step of the discrete adjoint of velocity Verlet in momentum
  * form (torch_sim/integrators/nve.py:74-130): xbar += vjp;
  * pbar += (dt / m) xbar; w = wscale * pbar (the next force-VJP cotangent). */

@@ -98,3 +98,79 @@ def calc_kT(p, m, system_idx, n_sys):
     tot = torch.zeros(n_sys, dtype=p.dtype).index_add(0, system_idx, s)
     cnt = torch.bincount(system_idx, minlength=n_sys).to(p.dtype)
     return tot / (3.0 * cnt)
+
+
+# ------------------------------------------------------------- NPT (GJF)
+# Isotropic NPT Langevin with the logarithmic strain eps = ln(V/V0)/3 as the
+# cell coordinate (torch_sim/integrators/npt.py:653-1107): the
+# Gronbech-Jensen-Farago integrator on particles and on eps.
+
+def gjf_particle_position(x, v, f, m, dt, alpha, beta, scale):
+    """x' = s x + (2 s / (s + 1)) b dt (v + dt F / (2m) + beta / (2m)),
+    b = 1 / (1 + alpha dt / (2m)) (npt.py:838-885).  ``alpha``, ``scale``
+    per atom [N]; ``beta`` [N,3]."""
+    m2 = 2 * m.unsqueeze(-1)
+    b = 1 / (1 + (alpha * dt) / (2 * m))
+    c1 = scale.unsqueeze(-1) * x
+    c2 = (2 * scale / (scale + 1)) * b * dt
+    c3 = v + dt * f / m2 + beta / m2
+    return c1 + c2.unsqueeze(-1) * c3
+
+
+def gjf_particle_velocity(v, f_old, f_new, m, dt, alpha, beta):
+    """v' = a v + dt (a F_old + F_new) / (2m) + b beta / m with
+    a = (1 - alpha dt / (2m)) / (1 + alpha dt / (2m)) (npt.py:316-367)."""
+    m2 = 2 * m
+    a = ((1 - (alpha * dt) / m2) / (1 + (alpha * dt) / m2)).unsqueeze(-1)
+    b = (1 / (1 + (alpha * dt) / m2)).unsqueeze(-1)
+    return a * v + dt * ((a * f_old) + f_new) / m2.unsqueeze(-1) + b * beta / m.unsqueeze(-1)
+
+
+def gjf_strain_position(eps, eps_dot, force, q, dt, cell_alpha, beta):
+    """eps' = eps + b dt eps_dot + b dt^2 F / (2Q) + b dt beta / (2Q),
+    b = 1 / (1 + c_alpha dt / (2Q)) (npt.py:769-798)."""
+    q2 = 2 * q
+    b = 1 / (1 + (cell_alpha * dt) / q2)
+    return eps + b * dt * eps_dot + b * dt * dt * force / q2 + b * dt * beta / q2
+
+
+def gjf_strain_velocity(eps_dot, f_old, f_new, q, dt, cell_alpha, beta):
+    """eps_dot' = a eps_dot + dt (a F_old + F_new) / (2Q) + b beta / Q
+    (npt.py:801-835)."""
+    a = (1 - (cell_alpha * dt) / (2 * q)) / (1 + (cell_alpha * dt) / (2 * q))
+    b = 1 / (1 + (cell_alpha * dt) / (2 * q))
+    return a * eps_dot + dt * ((a * f_old) + f_new) / (2 * q) + b * beta / q
+
+
+def strain_force(stress, volume, n_atoms, kT, p_ext):
+    """F_eps = 3 V (P - P_ext), P = -tr(sigma)/3 + n kT / V (npt.py:704-749)."""
+    p_vir = -torch.einsum("nii->n", stress) / 3
+    return 3 * volume * (p_vir + n_atoms * kT / volume - p_ext)
+
+
+def npt_langevin_isotropic_step(x, p, f, m, system_idx, ref_cell, eps, eps_dot, stress,
+                                force_fn, dt, kT, p_ext, alpha, cell_alpha, b_tau, cell_noise,
+                                noise):
+    """One isotropic NPT Langevin step (npt.py:988-1107) with the noise
+    supplied: ``cell_noise`` [S] and ``noise`` [N,3] standard normals, in the
+    reference's draw order.  Per-system tensors: kT, p_ext, alpha,
+    cell_alpha, b_tau [S]; ref_cell, stress [S,3,3] (column cells).
+    ``force_fn(x, cell) -> (energy, forces, stress)``.  Returns
+    (x, p, f, energy, stress, cell, eps, eps_dot)."""
+    n_sys = ref_cell.shape[0]
+    cnt = torch.bincount(system_idx, minlength=n_sys).to(x.dtype)
+    q = (cnt + 1) * kT * b_tau * b_tau
+    v0 = torch.linalg.det(ref_cell)
+    f_eps_n = strain_force(stress, v0 * torch.exp(3 * eps), cnt, kT, p_ext)
+    cell_beta = torch.sqrt(2.0 * cell_alpha * kT * dt) * cell_noise
+    beta = torch.sqrt(2 * alpha[system_idx] * kT[system_idx] * dt).unsqueeze(-1) * noise
+    eps_new = gjf_strain_position(eps, eps_dot, f_eps_n, q, dt, cell_alpha, cell_beta)
+    cell = torch.exp(eps_new)[:, None, None] * ref_cell
+    scale = torch.exp(eps_new - eps)[system_idx]
+    v = p / m.unsqueeze(-1)
+    x = gjf_particle_position(x, v, f, m, dt, alpha[system_idx], beta, scale)
+    e, f_new, stress_new = force_fn(x, cell)
+    f_eps_new = strain_force(stress_new, v0 * torch.exp(3 * eps_new), cnt, kT, p_ext)
+    eps_dot = gjf_strain_velocity(eps_dot, f_eps_n, f_eps_new, q, dt, cell_alpha, cell_beta)
+    v = gjf_particle_velocity(v, f, f_new, m, dt, alpha[system_idx], beta)
+    return x, v * m.unsqueeze(-1), f_new, e, stress_new, cell, eps_new, eps_dot

@@ -206,3 +206,127 @@ def test_initialized_momenta_have_the_requested_temperature():
     p = omd.initialize_momenta(m, idx, kT, torch.randn(n, 3, generator=g, dtype=F64))
     kt = omd.calc_kT(p, m, idx, 1).item()
     assert abs(kt / (0.0862 * (n - 1) / n) - 1.0) < 3e-3
+
+
+# ----------------------------------------------------------- NPT (GJF)
+def test_gjf_particle_update_samples_the_harmonic_distribution_at_large_dt():
+    """Gronbech-Jensen and Farago (Mol. Phys. 111, 983 (2013)): their
+    Langevin integrator gives the exact configurational distribution of a
+    harmonic well at any stable time step, <k x^2> = kT.  20,000
+    oscillators (omega = 1) at dt = 0.5 with friction 0.5: a wrong damping
+    factor a or b moves <k x^2> by tens of percent."""
+    from oracle import md as omd
+
+    g = torch.Generator().manual_seed(11)
+    n, k, kT, dt, alpha = 20000, 1.0, 1.0, 0.5, 0.5
+    m = torch.ones(n, dtype=F64)
+    a_t = torch.full((n,), alpha, dtype=F64)
+    x = torch.zeros(n, 3, dtype=F64)
+    v = torch.zeros(n, 3, dtype=F64)
+    f = -k * x
+    acc, n_acc = 0.0, 0
+    for step in range(600):
+        beta = math.sqrt(2 * alpha * kT * dt) * torch.randn(n, 3, generator=g, dtype=F64)
+        x = omd.gjf_particle_position(x, v, f, m, dt, a_t, beta, torch.ones(n, dtype=F64))
+        f_new = -k * x
+        v = omd.gjf_particle_velocity(v, f, f_new, m, dt, a_t, beta)
+        f = f_new
+        if step >= 300:
+            acc += (k * x * x).mean().item()
+            n_acc += 1
+    assert abs(acc / n_acc / kT - 1.0) < 0.03, acc / n_acc
+
+
+def test_gjf_strain_update_samples_the_harmonic_distribution():
+    """The strain coordinate's GJF update in a harmonic well F = -K eps:
+    <K eps^2> = kT for 20,000 independent cells."""
+    from oracle import md as omd
+
+    g = torch.Generator().manual_seed(12)
+    n, big_k, kT, dt = 20000, 4.0, 0.5, 0.3
+    q = torch.full((n,), 2.0, dtype=F64)
+    ca = torch.full((n,), 1.5, dtype=F64)
+    eps = torch.zeros(n, dtype=F64)
+    eps_dot = torch.zeros(n, dtype=F64)
+    fo = -big_k * eps
+    acc, n_acc = 0.0, 0
+    for step in range(800):
+        beta = torch.sqrt(2.0 * ca * kT * dt) * torch.randn(n, generator=g, dtype=F64)
+        eps = omd.gjf_strain_position(eps, eps_dot, fo, q, dt, ca, beta)
+        fn = -big_k * eps
+        eps_dot = omd.gjf_strain_velocity(eps_dot, fo, fn, q, dt, ca, beta)
+        fo = fn
+        if step >= 400:
+            acc += (big_k * eps * eps).mean().item()
+            n_acc += 1
+    assert abs(acc / n_acc / kT - 1.0) < 0.03, acc / n_acc
+
+
+def test_npt_step_without_forces_or_noise_scales_isotropically():
+    """Zero forces, momenta and noise, no friction: the strain moves by the
+    closed form b dt^2 F_eps / (2Q) with F_eps = 3 V (P - P_ext), the cell
+    becomes exp(eps') h0 and the positions keep their fractional
+    coordinates."""
+    from oracle import md as omd
+
+    g = torch.Generator().manual_seed(13)
+    n_sys = 2
+    idx = torch.tensor([0] * 5 + [1] * 7)
+    ref = torch.stack([torch.eye(3, dtype=F64) * 9.0,
+                       torch.tensor([[8.0, 0.5, 0.0], [0.0, 9.0, 0.3], [0.2, 0.0, 10.0]],
+                                    dtype=F64)])
+    frac = torch.rand(12, 3, generator=g, dtype=F64)
+    x = torch.einsum("nij,nj->ni", ref[idx], frac)
+    stress = torch.stack([torch.eye(3, dtype=F64) * -0.01, torch.eye(3, dtype=F64) * 0.02])
+    zeros = torch.zeros(12, 3, dtype=F64)
+    m = torch.full((12,), 2.0, dtype=F64)
+    kT = torch.tensor([0.05, 0.08], dtype=F64)
+    p_ext = torch.tensor([0.001, -0.002], dtype=F64)
+    b_tau = torch.tensor([3.0, 4.0], dtype=F64)
+    zero_s = torch.zeros(n_sys, dtype=F64)
+    force_fn = lambda xx, cc: (zero_s, torch.zeros_like(xx), stress)  # noqa: E731
+    dt = 0.2
+    out = omd.npt_langevin_isotropic_step(
+        x, zeros, zeros, m, idx, ref, zero_s, zero_s, stress, force_fn, dt, kT, p_ext,
+        torch.zeros(n_sys, dtype=F64), torch.zeros(n_sys, dtype=F64), b_tau,
+        torch.zeros(n_sys, dtype=F64), zeros)
+    x1, p1, _, _, _, cell, eps, eps_dot = out
+    n_at = torch.tensor([5.0, 7.0], dtype=F64)
+    vol = torch.linalg.det(ref)
+    f_eps = 3 * vol * (-torch.diagonal(stress, dim1=1, dim2=2).sum(1) / 3 + n_at * kT / vol - p_ext)
+    q = (n_at + 1) * kT * b_tau ** 2
+    torch.testing.assert_close(eps, dt * dt * f_eps / (2 * q), rtol=1e-14, atol=0)
+    torch.testing.assert_close(cell, torch.exp(eps)[:, None, None] * ref, rtol=1e-15, atol=0)
+    frac1 = torch.linalg.solve(cell[idx], x1.unsqueeze(-1)).squeeze(-1)
+    torch.testing.assert_close(frac1, frac, rtol=0, atol=1e-14)
+    assert p1.abs().max() == 0.0
+    # the strain velocity averages the force at the old and the new volume
+    vol1 = vol * torch.exp(3 * eps)
+    f_eps1 = 3 * vol1 * (-torch.diagonal(stress, dim1=1, dim2=2).sum(1) / 3 + n_at * kT / vol1
+                         - p_ext)
+    torch.testing.assert_close(eps_dot, dt * (f_eps + f_eps1) / (2 * q), rtol=1e-13, atol=0)
+
+
+def test_strain_force_of_a_perfect_lattice_closed_form():
+    """F_eps = 3 V (P - P_ext) with the virial pressure of a compressed fcc
+    Cu crystal from its shell sum, P_vir = -N / (6V) sum_s n_s r_s V'(r_s),
+    plus the kinetic term N kT / V."""
+    from oracle import md as omd
+
+    a, rc = 3.50, 6.0
+    pos = _fcc(a, (3, 3, 3))
+    n = pos.shape[0]
+    cell = (torch.eye(3, dtype=F64) * 3 * a).unsqueeze(0)
+    fn = lambda d, zi, zj: om.morse_pair(d, zi, zj, **MORSE)  # noqa: E731
+    out = om.pair_forward(pos, cell, torch.tensor([True] * 3), torch.ones(n, dtype=torch.long),
+                          torch.zeros(n, dtype=torch.long), fn, rc)
+    vol = (3 * a) ** 3
+    shells = [(a * math.sqrt(k / 2), c) for k, c in SHELLS if a * math.sqrt(k / 2) < rc]
+    p_vir = -n / (6 * vol) * sum(c * r * _morse_dv(r) for r, c in shells)
+    kT, p_ext = 0.0862, 0.004
+    f = omd.strain_force(out["stress"], torch.tensor([vol], dtype=F64),
+                         torch.tensor([float(n)], dtype=F64), torch.tensor([kT], dtype=F64),
+                         torch.tensor([p_ext], dtype=F64))
+    expect = 3 * vol * (p_vir + n * kT / vol - p_ext)
+    assert abs(f.item() - expect) < 1e-10 * abs(expect)
+    assert p_vir > 0  # compressed: positive pressure

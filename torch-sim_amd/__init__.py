@@ -13,7 +13,11 @@ from .integrators import (
     INTEGRATOR_REGISTRY,
     Integrator,
     MDState,
+    NPTLangevinIsotropicState,
+    NPTState,
     initialize_momenta,
+    npt_langevin_isotropic_init,
+    npt_langevin_isotropic_step,
     momentum_step,
     nve_init,
     nve_step,

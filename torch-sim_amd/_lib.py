@@ -68,6 +68,15 @@ _SIGS = {
     ),
     "tsx_md_reduce": (I, [P, P, I, I, P, P, I, P, P, P]),
     "tsx_copy_ref": (I, [P, P, L, P, P, P]),
+    "tsx_npt_ws_bytes": (L, [L, I]),
+    "tsx_npt_langevin_iso_pre": (
+        I,
+        [I, P, P, L, P, P, P, P, P, P, P, P, P, D, P, P, P, P, P, P, P, P, P, P],
+    ),
+    "tsx_npt_langevin_iso_post": (
+        I,
+        [I, P, P, L, P, P, P, P, P, P, P, P, D, P, P, P, P, P, P, P],
+    ),
     "tsx_fire_ws_bytes": (L, [I]),
     "tsx_fire_pre": (
         I,

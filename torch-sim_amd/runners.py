@@ -107,7 +107,7 @@ def integrate(
                 kw[key] = kw[key] * factor
 
     engine_ok = (
-        isinstance(integrator, Integrator)
+        integrator in (Integrator.nve, Integrator.nvt_langevin)
         and isinstance(model, PairPotentialModel)
         and state.dtype == torch.float64
         and n_steps > 0
