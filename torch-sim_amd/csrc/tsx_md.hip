@@ -11,6 +11,8 @@
 // it is the last step -- runs the first half of the next step (B A O A for
 // Langevin, B A(dt) for NVE) into the other position buffer, checking the
 // displacement since the last list build against half the Verlet skin.
+#include <type_traits>
+
 #include "tsx_pair.h"
 
 namespace tsx {
@@ -195,20 +197,30 @@ __device__ __forceinline__ RowAcc accumulate_rel(const PairP* sp, int ns, int ti
   const uint32_t* wp = nbr + t * TPA + lane;
   const int64_t stride = ld * TPA;
   const int iters = cnt > lane ? (cnt - lane + TPA - 1) / TPA : 0;
-  for (int k0 = 0; k0 < iters; k0 += UNR_REL) {
+  // Branch-free batches: across a wave nearly every entry of a batch is in
+  // range on some lane, so the pair arithmetic runs for all of them anyway;
+  // per-entry branches would only add exec-mask traffic.  Only a lane's last
+  // batch can run past its row's end: there the lane re-reads its last entry
+  // (the load needs no guard) and the entries past the end add exact zeros,
+  // so the sums keep the bits of an entry-by-entry walk.
+  const uint32_t* wlast = wp + (int64_t)(iters - 1) * stride;  // used when iters > 0
+  const uint32_t* pk = wp;  // entry k0 of this lane
+  auto batch = [&](int k0, auto tail_tag) {
+    constexpr bool TAIL = decltype(tail_tag)::value;
     uint32_t w[UNR_REL];
 #pragma unroll
-    for (int u = 0; u < UNR_REL; ++u)
-      w[u] = (k0 + u < iters) ? (NTW ? __builtin_nontemporal_load(wp + (int64_t)(k0 + u) * stride)
-                                     : wp[(int64_t)(k0 + u) * stride])
-                              : 0u;
+    for (int u = 0; u < UNR_REL; ++u) {
+      const uint32_t* pa = (!TAIL || k0 + u < iters) ? pk : wlast;
+      w[u] = NTW ? __builtin_nontemporal_load(pa) : *pa;
+      pk += stride;
+    }
     float4 q[UNR_REL];
 #pragma unroll
     for (int u = 0; u < UNR_REL; ++u) q[u] = rel_s[w[u] & jmask];
     uint32_t amb = 0;  // entries within the rounding margin of rc
 #pragma unroll
     for (int u = 0; u < UNR_REL; ++u) {
-      if (k0 + u >= iters) break;
+      const bool valid = !TAIL || k0 + u < iters;
       const float fx = (float)nl_shift(w[u], jbits, sbits, 0);
       const float fy = (float)nl_shift(w[u], jbits, sbits, 1);
       const float fz = (float)nl_shift(w[u], jbits, sbits, 2);
@@ -218,14 +230,12 @@ __device__ __forceinline__ RowAcc accumulate_rel(const PairP* sp, int ns, int ti
       const float dy = (q[u].y - yri) + sy;
       const float dz = (q[u].z - zri) + sz;
       const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-      if (!(d2 < hi2)) continue;
-      if (!(d2 < lo2)) {
-        amb |= 1u << u;
-        continue;
-      }
+      const bool in = valid && d2 < lo2;
+      amb |= (valid && !(d2 < lo2) && d2 < hi2) ? (1u << u) : 0u;
       float e, fs;
-      pair_eval<KIND>(d2, MULTI ? sp[ti * ns + __float_as_int(q[u].w)] : p0, e, fs);
-      add_pair<VIRIAL>(a, e, fs, dx, dy, dz);
+      // a discarded entry may have d2 = 0 (inf / NaN here): the select drops it
+      pair_eval_pos<KIND>(d2, MULTI ? sp[ti * ns + __float_as_int(q[u].w)] : p0, e, fs);
+      add_pair<VIRIAL>(a, in ? e : 0.f, in ? fs : 0.f, dx, dy, dz);
     }
     while (amb) {  // rare: the FP64 displacement decides
       const int u = __builtin_ctz(amb);
@@ -249,7 +259,11 @@ __device__ __forceinline__ RowAcc accumulate_rel(const PairP* sp, int ns, int ti
                       MULTI ? sp[ti * ns + (int)__double_as_longlong(pj.w)] : p0, e, fs);
       add_pair<VIRIAL>(a, e, fs, (float)gx, (float)gy, (float)gz);
     }
-  }
+  };
+  const int full = iters - iters % UNR_REL;  // entries in whole batches
+  int k0 = 0;
+  for (; k0 < full; k0 += UNR_REL) batch(k0, std::false_type{});
+  if (k0 < iters) batch(k0, std::true_type{});
 #pragma unroll
   for (int off = TPA / 2; off > 0; off >>= 1) {
     a.fx += __shfl_xor(a.fx, off, 64);

@@ -91,6 +91,44 @@ __device__ __forceinline__ void pair_eval(float r2, const PairP& p, float& e, fl
   }
 }
 
+// pair_eval for r2 > 0 without branches: the same operations, hence the same
+// bits, as pair_eval where pair_eval finds r2 > 0.  Loops that evaluate every
+// entry of a batch and discard some with a select call it with a positive
+// stand-in r2 for those.
+template <int KIND>
+__device__ __forceinline__ void pair_eval_pos(float r2, const PairP& p, float& e, float& fs) {
+  if (KIND == TSX_LJ) {
+    const float ir2 = frcp(r2);
+    const float sr2 = p.a * ir2;
+    const float sr6 = sr2 * sr2 * sr2;
+    e = p.b * (sr6 * sr6 - sr6);
+    fs = p.c * (2.f * sr6 * sr6 - sr6) * ir2;
+  } else if (KIND == TSX_MORSE) {
+    const float ir = frsq(r2);
+    const float r = r2 * ir;
+    const float x = __expf(-p.c * (r - p.a));
+    const float omx = 1.f - x;
+    e = p.b * omx * omx - p.b;
+    fs = -p.d * x * omx * ir;
+  } else {
+    const float ir = frsq(r2);
+    const float r = r2 * ir;
+    const float t = 1.f - r * p.e;
+    float ee, ff;
+    if (p.c == 2.f) {
+      ee = 0.5f * p.b * t * t;
+      ff = p.d * t * ir;
+    } else {
+      const float tm1 = powf(t, p.c - 1.f);
+      ee = p.b / p.c * tm1 * t;
+      ff = p.d * tm1 * ir;
+    }
+    const bool inside = r < p.a;
+    e = inside ? ee : 0.f;
+    fs = inside ? ff : 0.f;
+  }
+}
+
 // Second radial derivative helpers for the backward kernels:
 //   phi'(r) and phi''(r) in FP32 together with dphi'/dsigma, dphi'/deps.
 template <int KIND>
