@@ -67,6 +67,12 @@ extern "C" {
 #define TSX_LJ 0          /* 4 eps [(s/r)^12 - (s/r)^6]                  */
 #define TSX_MORSE 1       /* eps (1 - exp(-a (r - s)))^2 - eps            */
 #define TSX_SOFT_SPHERE 2 /* eps/a (1 - r/s)^a for r < s                  */
+/* particle life: a force without an energy (tsx_pair_forces only), with
+ * (sigma, epsilon, alpha) = (outer radius s, amplitude A, inner radius b):
+ *   f(r) = r/b - 1 (r < b),  A (1 - |2r - 1 - b| / (1 - b)) (b <= r < s),  0
+ * and the force on j due to i f(r) (r_j - r_i) / r
+ * (torch_sim/models/particle_life.py:24-56, pair_potential.py:467-612) */
+#define TSX_PARTICLE_LIFE 3
 
 /* Library version (major*10000 + minor*100 + patch). */
 int tsx_version(void);

@@ -50,8 +50,25 @@ PINNING = {
     "energy is the full shell sum, the stress is N/(6V) sum_s n_s r_s V'(r_s) I and each "
     "atom carries 1/N of it (at a = 3.61 and compressed 3.50); on a jittered triclinic batch "
     "sum energies = 2E and sum stresses = the system stress per system",
+    "models.particle_life_force": "test_particle_life.py:13-35 (inside beta negative, zero "
+    "from sigma on, linear in A) and the closed form: r/beta - 1, the tent's zero at beta "
+    "and its peak A at (1 + beta) / 2",
+    "models.pair_forces_forward": "non-periodic dimer written out (forces -2f r_hat / +2f "
+    "r_hat on a full list, -f / +f on a half list, stress -(f r / V) x x on both); with "
+    "f = -dV/dr of Morse on a jittered Cu batch the half-list forces, stress and per-atom "
+    "stresses equal pair_forward's and the full list doubles the forces only",
     "md.*": "BAOAB one-step closed form with supplied noise; NVE energy drift; "
     "COM removal; batch==single",
+    "md.gjf_particle_position / gjf_particle_velocity": "harmonic oscillator at a large dt: "
+    "the GJF update samples <x^2> = kT / k and <v^2> = kT / m (the property that fixes its "
+    "coefficients a, b)",
+    "md.gjf_strain_position / gjf_strain_velocity": "harmonic strain potential: <eps^2> = "
+    "kT / k and <eps_dot^2> = kT / Q",
+    "md.strain_force": "perfect lattice: 3 V (P_virial + n kT / V - P_ext) with the "
+    "lattice's closed-form virial pressure",
+    "md.npt_langevin_isotropic_step": "without forces and noise the step is an isotropic "
+    "scaling: positions scale by exp(eps' - eps), the cell is exp(eps') h0, the strain "
+    "velocity follows the kinetic pressure alone",
     "md.kinetic_energy / md.calc_kT": "constructed momenta p = sqrt(m) u give KE = 0.5 sum "
     "|u|^2 and kT = sum |u|^2 / (3n) per system; Maxwell-Boltzmann momenta at kT with the COM "
     "removed measure kT (n-1)/n",

@@ -1,8 +1,9 @@
 """Oracle pair potentials (CPU, float64).  Test infrastructure only.
 
-Restates torch_sim/models/{lennard_jones,morse,soft_sphere,pair_potential}.py:
-the closed-form pair energies and the half-list forward pass with autograd
-forces and the pair-virial stress.
+Restates torch_sim/models/{lennard_jones,morse,soft_sphere,pair_potential,
+particle_life}.py: the closed-form pair energies, the particle-life force,
+the half-list forward pass with autograd forces and the pair-virial stress,
+and the force-defined forward pass of PairForcesModel.
 """
 
 from __future__ import annotations
@@ -136,4 +137,49 @@ def pair_forward(
             out["stresses"] = sa / vol[system_idx, None, None]
     if not create_graph:
         out = {k: v.detach() for k, v in out.items()}
+    return out
+
+
+def particle_life_force(dr, zi, zj, A=1.0, beta=0.3, sigma=1.0):  # noqa: N803
+    """Particle-life force magnitude (particle_life.py:24-56)."""
+    zero = torch.zeros_like(dr)
+    inner = torch.where(dr < beta, dr / beta - 1.0, zero)
+    outer = A * (1.0 - torch.abs(2.0 * dr - 1.0 - beta) / (1.0 - beta))
+    return inner + torch.where((dr >= beta) & (dr < sigma), outer, zero)
+
+
+def pair_forces_forward(positions, cell_col, pbc, atomic_numbers, system_idx, force_fn, cutoff,
+                        *, half=False, compute_stress=True, per_atom=False, nl=None):
+    """PairForcesModel.forward (pair_potential.py:539-612): energies zero;
+    every list entry adds -f r_hat to its first atom and +f r_hat to its
+    second (on a full list each pair twice); the stress weights of
+    _accumulate_stress (pair_potential.py:236-274)."""
+    n_sys = cell_col.shape[0]
+    row_cell = cell_col.transpose(1, 2)
+    pbc_b = pbc.unsqueeze(0).expand(n_sys, -1) if pbc.ndim == 1 else pbc
+    if nl is None:
+        nl = onl.brute_force_nl
+    mapping, sysm, shifts = nl(positions, row_cell, pbc_b, cutoff, system_idx)
+    if half:
+        mapping, sysm, shifts = onl.full_to_half_list(mapping, sysm, shifts)
+    cell_shifts = torch.einsum("jn,jnm->jm", shifts.to(positions.dtype), row_cell[sysm])
+    dr = positions[mapping[1]] - positions[mapping[0]] + cell_shifts
+    dist = dr.norm(dim=1)
+    f = force_fn(dist, atomic_numbers[mapping[0]], atomic_numbers[mapping[1]])
+    f = torch.where(dist < cutoff, f, torch.zeros_like(f))
+    safe = torch.where(dist > 0, dist, torch.ones_like(dist))
+    fvec = (f / safe)[:, None] * dr
+    forces = torch.zeros_like(positions).index_add(0, mapping[0], -fvec)
+    forces = forces.index_add(0, mapping[1], fvec)
+    out = {"energy": torch.zeros(n_sys, dtype=positions.dtype), "forces": forces}
+    if compute_stress:
+        vol = torch.abs(torch.linalg.det(row_cell))
+        spp = torch.einsum("...i,...j->...ij", dr, fvec)
+        stress = torch.zeros((n_sys, 3, 3), dtype=positions.dtype).index_add(0, sysm, -spp)
+        out["stress"] = stress / vol[:, None, None] * (1.0 if half else 0.5)
+        if per_atom:
+            w = 0.5 if half else 0.25
+            sa = torch.zeros((positions.shape[0], 3, 3), dtype=positions.dtype)
+            sa = sa.index_add(0, mapping[0], -w * spp).index_add(0, mapping[1], -w * spp)
+            out["stresses"] = sa / vol[system_idx, None, None]
     return out

@@ -330,3 +330,73 @@ def test_strain_force_of_a_perfect_lattice_closed_form():
     expect = 3 * vol * (p_vir + n * kT / vol - p_ext)
     assert abs(f.item() - expect) < 1e-10 * abs(expect)
     assert p_vir > 0  # compressed: positive pressure
+
+
+def test_particle_life_force_known_answers():
+    """test_particle_life.py:13-35 and the closed form of the force."""
+    z = torch.ones(6, dtype=torch.long)
+    r = torch.tensor([0.1, 0.2, 0.3, 0.65, 1.0, 1.5], dtype=F64)
+    f = om.particle_life_force(r, z, z, A=2.0, beta=0.3, sigma=1.0)
+    assert (f[:2] < 0).all() and (f[4:] == 0).all()
+    torch.testing.assert_close(f[:2], r[:2] / 0.3 - 1.0, rtol=0, atol=1e-15)
+    assert abs(f[2].item()) < 1e-15 and abs(f[3].item() - 2.0) < 1e-15
+    f1 = om.particle_life_force(r, z, z, A=1.0, beta=0.3, sigma=1.0)
+    torch.testing.assert_close(om.particle_life_force(r, z, z, A=3.0, beta=0.3, sigma=1.0)[3:],
+                               3.0 * f1[3:])
+
+
+def test_pair_forces_forward_dimer_closed_form():
+    """Two atoms 0.5 apart along (1, 2, 2)/3 in a non-periodic box: f = A (1 -
+    0.3 / 0.7); a full list applies every entry's force and reaction (twice
+    per pair), a half list once; the stress is -(f r / V) r_hat r_hat."""
+    d = 0.5 * torch.tensor([1.0, 2.0, 2.0], dtype=F64) / 3.0
+    x = torch.stack([torch.tensor([1.0, 1.0, 1.0], dtype=F64), torch.tensor([1.0, 1.0, 1.0],
+                                                                            dtype=F64) + d])
+    cell = 10.0 * torch.eye(3, dtype=F64).unsqueeze(0)
+    pbc = torch.zeros(3, dtype=torch.bool)
+    z = torch.ones(2, dtype=torch.long)
+    idx = torch.zeros(2, dtype=torch.long)
+    fn = lambda r, zi, zj: om.particle_life_force(r, zi, zj, A=1.5, beta=0.3, sigma=1.0)  # noqa: E731
+    fmag = 1.5 * (1.0 - 0.3 / 0.7)
+    rhat = d / 0.5
+    for half, k in ((False, 2.0), (True, 1.0)):
+        out = om.pair_forces_forward(x, cell, pbc, z, idx, fn, 1.0, half=half, per_atom=True)
+        torch.testing.assert_close(out["forces"][0], -k * fmag * rhat, rtol=0, atol=1e-14)
+        torch.testing.assert_close(out["forces"][1], k * fmag * rhat, rtol=0, atol=1e-14)
+        sig = -(fmag * 0.5 / 1000.0) * torch.outer(rhat, rhat)
+        torch.testing.assert_close(out["stress"][0], sig, rtol=0, atol=1e-16)
+        torch.testing.assert_close(out["stresses"].sum(0), sig, rtol=0, atol=1e-16)
+        assert out["energy"].tolist() == [0.0]
+
+
+def test_pair_forces_forward_with_minus_dv_dr_equals_pair_forward():
+    """force_fn = -dV/dr (Morse): on a half list the force model's forces,
+    stress and per-atom stresses equal the potential model's; the full list
+    doubles the forces (every pair applied twice) and keeps the stresses."""
+    g = torch.Generator().manual_seed(41)
+    pos = torch.cat([_fcc(3.61, (2, 2, 3)), _fcc(3.61, (3, 2, 2))])
+    pos = pos + 0.08 * torch.randn(96, 3, generator=g, dtype=F64)
+    cell = torch.stack([
+        torch.tensor([[7.22, 0.3, 0.0], [0.0, 7.22, 0.2], [0.1, 0.0, 10.83]], dtype=F64),
+        torch.tensor([[10.83, 0.0, -0.2], [0.4, 7.22, 0.0], [0.0, 0.0, 7.22]], dtype=F64)])
+    z, idx = torch.ones(96, dtype=torch.long), torch.arange(2).repeat_interleave(48)
+    pbc = torch.tensor([True] * 3)
+    prm = MORSE
+
+    def force_fn(r, zi, zj):
+        rg = r.detach().requires_grad_()
+        (dv,) = torch.autograd.grad(om.morse_pair(rg, zi, zj, **prm).sum(), rg)
+        return -dv
+
+    pot = om.pair_forward(pos, cell, pbc, z, idx,
+                          lambda r, zi, zj: om.morse_pair(r, zi, zj, **prm), 6.0, per_atom=True)
+    args = (pos, cell, pbc, z, idx, force_fn, 6.0)
+    hf = om.pair_forces_forward(*args, half=True, per_atom=True)
+    ff = om.pair_forces_forward(*args, half=False, per_atom=True)
+    scale = pot["forces"].abs().max().item()
+    assert scale > 0.1
+    torch.testing.assert_close(hf["forces"], pot["forces"], rtol=0, atol=1e-12 * scale)
+    torch.testing.assert_close(ff["forces"], 2.0 * pot["forces"], rtol=0, atol=2e-12 * scale)
+    for out in (hf, ff):
+        torch.testing.assert_close(out["stress"], pot["stress"], rtol=1e-12, atol=1e-15)
+        torch.testing.assert_close(out["stresses"], pot["stresses"], rtol=1e-12, atol=1e-15)

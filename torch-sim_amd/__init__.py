@@ -32,11 +32,16 @@ from .models import (
     MultiLennardJonesModel,
     MultiLennardJonesPairFn,
     MultiSoftSpherePairFn,
+    PairForcesModel,
     PairPotentialModel,
+    ParticleLifeModel,
+    SerialSumModel,
     SoftSphereModel,
     SoftSphereMultiModel,
+    SumModel,
     lennard_jones_pair,
     morse_pair,
+    particle_life_pair_force,
     soft_sphere_pair,
 )
 from .neighbors import default_batched_nl, torch_nl_linked_cell, torch_nl_n2, torchsim_nl

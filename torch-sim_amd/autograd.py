@@ -36,6 +36,9 @@ class PairTerms:
     taken from a :class:`~torch_sim_amd.models.PairPotentialModel`."""
 
     def __init__(self, model, *, tpa: int | None = None, skin: float = 0.0):
+        if not getattr(model, "fused_kernels", False):
+            raise TypeError("the backward kernels take the closed-form pair potentials "
+                            "(LJ, Morse, soft sphere and their species-pair forms)")
         self.model = model
         self.kind = model.kind
         self.n_species = model.n_species

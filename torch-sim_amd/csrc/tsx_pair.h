@@ -10,6 +10,8 @@ namespace tsx {
 //   LJ:          a = sigma^2, b = 4 eps,  c = 24 eps
 //   Morse:       a = sigma,   b = eps,    c = alpha,  d = 2 eps alpha
 //   soft sphere: a = sigma,   b = eps,    c = alpha,  d = eps / sigma
+//   particle life (sigma, eps, alpha = outer radius s, amplitude A, inner
+//   radius b): a = b, b = A, c = s, d = 1 / b, e = 1 / (1 - b)
 struct PairP {
   float a, b, c, d, e;
 };
@@ -18,7 +20,13 @@ __host__ __device__ inline PairP make_pair_params(int kind, double sigma, double
                                                   double alpha) {
   PairP p;
   p.e = (float)(1.0 / sigma);
-  if (kind == TSX_LJ) {
+  if (kind == TSX_PARTICLE_LIFE) {
+    p.a = (float)alpha;
+    p.b = (float)eps;
+    p.c = (float)sigma;
+    p.d = (float)(1.0 / alpha);
+    p.e = (float)(1.0 / (1.0 - alpha));
+  } else if (kind == TSX_LJ) {
     p.a = (float)(sigma * sigma);
     p.b = (float)(4.0 * eps);
     p.c = (float)(24.0 * eps);
@@ -71,6 +79,18 @@ __device__ __forceinline__ void pair_eval(float r2, const PairP& p, float& e, fl
       e = 0.f;
       fs = 0.f;
     }
+  } else if (KIND == TSX_PARTICLE_LIFE) {
+    e = 0.f;
+    if (r2 > 0.f) {
+      const float ir = frsq(r2);
+      const float r = r2 * ir;
+      const float f = r < p.a ? r * p.d - 1.f
+                    : r < p.c ? p.b * (1.f - fabsf(2.f * r - 1.f - p.a) * p.e)
+                              : 0.f;
+      fs = f * ir;
+    } else {
+      fs = 0.f;
+    }
   } else {
     const float ir = r2 > 0.f ? frsq(r2) : 0.f;
     const float r = r2 * ir;
@@ -110,6 +130,14 @@ __device__ __forceinline__ void pair_eval_pos(float r2, const PairP& p, float& e
     const float omx = 1.f - x;
     e = p.b * omx * omx - p.b;
     fs = -p.d * x * omx * ir;
+  } else if (KIND == TSX_PARTICLE_LIFE) {
+    const float ir = frsq(r2);
+    const float r = r2 * ir;
+    const float f = r < p.a ? r * p.d - 1.f
+                  : r < p.c ? p.b * (1.f - fabsf(2.f * r - 1.f - p.a) * p.e)
+                            : 0.f;
+    e = 0.f;
+    fs = f * ir;
   } else {
     const float ir = frsq(r2);
     const float r = r2 * ir;

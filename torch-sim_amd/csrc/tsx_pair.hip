@@ -249,7 +249,7 @@ int tsx_pair_forces(int32_t kind, const double* params, int32_t n_species,
                     int64_t n_atoms, int32_t jbits, int32_t tpa, int64_t ld, double cutoff,
                     int32_t half, double* forces, double* e_atom, double* w_atom,
                     void* stream) {
-  TSX_REQUIRE(kind >= TSX_LJ && kind <= TSX_SOFT_SPHERE, "unknown pair kind %d", kind);
+  TSX_REQUIRE(kind >= TSX_LJ && kind <= TSX_PARTICLE_LIFE, "unknown pair kind %d", kind);
   TSX_REQUIRE(n_species >= 1 && n_species <= MAX_SPECIES, "n_species must be in [1, %d]",
               MAX_SPECIES);
   TSX_REQUIRE(n_species == 1 || species != nullptr || (pos_stride == 4 && !half),
@@ -277,8 +277,10 @@ int tsx_pair_forces(int32_t kind, const double* params, int32_t n_species,
     rc = dispatch_tpa<TSX_LJ>(TSX_ARGS);
   else if (kind == TSX_MORSE)
     rc = dispatch_tpa<TSX_MORSE>(TSX_ARGS);
-  else
+  else if (kind == TSX_SOFT_SPHERE)
     rc = dispatch_tpa<TSX_SOFT_SPHERE>(TSX_ARGS);
+  else
+    rc = dispatch_tpa<TSX_PARTICLE_LIFE>(TSX_ARGS);
 #undef TSX_ARGS
   TSX_REQUIRE(rc == 0, "unsupported launch configuration");
   TSX_CHECK_LAUNCH();

@@ -74,8 +74,9 @@ class BatchedMD:
         use_graph: bool = True,
         chunk_steps: int = 200,
     ):
-        if not isinstance(model, PairPotentialModel):
-            raise TypeError("BatchedMD runs the closed-form pair models of torch_sim_amd")
+        if not (isinstance(model, PairPotentialModel) and model.fused_kernels):
+            raise TypeError("BatchedMD runs the closed-form pair potentials of torch_sim_amd "
+                            "(LJ, Morse, soft sphere and their species-pair forms)")
         _lib.require_gpu(state.positions, "state.positions")
         if state.dtype != torch.float64:
             raise TypeError("BatchedMD integrates in float64")

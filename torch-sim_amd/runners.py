@@ -109,6 +109,7 @@ def integrate(
     engine_ok = (
         integrator in (Integrator.nve, Integrator.nvt_langevin)
         and isinstance(model, PairPotentialModel)
+        and model.fused_kernels
         and state.dtype == torch.float64
         and n_steps > 0
         and bool((kTs == kTs[0]).all())
@@ -193,6 +194,8 @@ def _gather_systems(parts, cls, template, counts_new: np.ndarray):
                                                output_size=n_new)
         elif name == "_rng":
             kw[name] = None
+        elif name in ("_atom_extras", "_system_extras"):
+            kw[name] = {}
         elif isinstance(v0, torch.Tensor) and name in cls._atom_attributes:
             kw[name] = torch.cat([p[0].__dict__[name].index_select(0, p[2]) for p in parts
                                   if p[3]]).contiguous()
@@ -220,7 +223,8 @@ def _scatter_systems(dst, src, dst_ids, dst_rows, src_ids, src_rows) -> None:
 
 
 def _resident_ok(model, init_fn, step_fn) -> bool:
-    return isinstance(model, PairPotentialModel) and init_fn is fire_init and step_fn is fire_step
+    return (isinstance(model, PairPotentialModel) and model.fused_kernels
+            and init_fn is fire_init and step_fn is fire_step)
 
 
 def _optimize_resident(state0, model, *, convergence_fn, max_steps, steps_between_swaps,

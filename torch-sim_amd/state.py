@@ -30,6 +30,10 @@ class SimState:
     atomic_numbers: torch.Tensor
     system_idx: torch.Tensor | None = None
     _rng: Any = field(default=None, repr=False)
+    # model outputs kept on the state (store_model_extras, state.py:376-404):
+    # per-system and per-atom tensors, read back as attributes
+    _system_extras: dict = field(default_factory=dict, repr=False)
+    _atom_extras: dict = field(default_factory=dict, repr=False)
 
     _atom_attributes: ClassVar[set[str]] = {
         "positions",
@@ -39,8 +43,37 @@ class SimState:
     }
     _system_attributes: ClassVar[set[str]] = {"cell"}
     _global_attributes: ClassVar[set[str]] = {"pbc", "_rng"}
+    _extras_names: ClassVar[tuple[str, str]] = ("_system_extras", "_atom_extras")
+
+    def __getattr__(self, name: str):
+        """Extras read as attributes (state.py:337-361); only reached when
+        the ordinary lookup fails."""
+        if not name.startswith("_"):
+            for ex in SimState._extras_names:
+                try:
+                    d = object.__getattribute__(self, ex)
+                except AttributeError:
+                    continue
+                if name in d:
+                    return d[name]
+        raise AttributeError(f"'{type(self).__name__}' has no attribute or extra '{name}'")
+
+    def __setattr__(self, name: str, value) -> None:
+        """A write to an existing extra updates it (None removes it)."""
+        if not name.startswith("_"):
+            for ex in SimState._extras_names:
+                d = self.__dict__.get(ex)
+                if d is not None and name in d:
+                    if value is None:
+                        del d[name]
+                    else:
+                        d[name] = value
+                    return
+        object.__setattr__(self, name, value)
 
     def __post_init__(self) -> None:
+        self._system_extras = dict(self._system_extras or {})
+        self._atom_extras = dict(self._atom_extras or {})
         if not isinstance(self.pbc, torch.Tensor):
             v = [self.pbc] * 3 if isinstance(self.pbc, bool) else list(self.pbc)
             self.pbc = torch.tensor(v, dtype=torch.bool, device=self.positions.device)
@@ -112,6 +145,17 @@ class SimState:
     def rng(self, value: Any) -> None:
         self._rng = value
 
+    @property
+    def system_extras(self) -> dict:
+        return self._system_extras
+
+    @property
+    def atom_extras(self) -> dict:
+        return self._atom_extras
+
+    def has_extras(self, key: str) -> bool:
+        return key in self._system_extras or key in self._atom_extras
+
     def get_number_of_degrees_of_freedom(self) -> torch.Tensor:
         """3 n per system (no constraints; constraints.py:691-699)."""
         return 3 * self.n_atoms_per_system
@@ -126,6 +170,8 @@ class SimState:
             v = getattr(self, name)
             if isinstance(v, torch.Tensor):
                 kw[name] = v.clone()
+            elif name in self._extras_names:
+                kw[name] = {k: t.clone() for k, t in v.items()}
             elif name == "_rng":
                 kw[name] = v if not isinstance(v, torch.Generator) else None
             else:
@@ -146,6 +192,8 @@ class SimState:
                     v = v.to(device=device, dtype=dtype)
                 else:
                     v = v.to(device=device)
+            elif name in self._extras_names:
+                v = {k: t.to(device=device) for k, t in v.items()}
             elif name == "_rng":
                 v = None if isinstance(v, torch.Generator) else v
             kw[name] = v
@@ -161,7 +209,20 @@ class SimState:
         self.cell = cell
 
     def store_model_extras(self, model_output: dict[str, torch.Tensor]) -> None:
-        """Accepted for interface compatibility (state.py:376); extras are ignored."""
+        """Keep the model outputs other than energy / forces / stress
+        (state.py:376-404): leading dimension n_atoms -> per-atom extras
+        (also when n_atoms == n_systems), n_systems -> per-system extras;
+        scalars and other shapes are skipped."""
+        n, s = self.n_atoms, self.n_systems
+        for key, val in model_output.items():
+            if key in ("energy", "forces", "stress"):
+                continue
+            if not isinstance(val, torch.Tensor) or val.ndim == 0:
+                continue
+            if val.shape[0] == n:
+                self._atom_extras[key] = val
+            elif val.shape[0] == s:
+                self._system_extras[key] = val
 
     @classmethod
     def from_state(cls, state: SimState, **kwargs: Any):
@@ -182,6 +243,10 @@ class SimState:
                     v = torch.zeros_like(v[a : a + c]) if name == "system_idx" else v[a : a + c]
                 elif name in self._system_attributes and isinstance(v, torch.Tensor):
                     v = v[s : s + 1]
+                elif name == "_atom_extras":
+                    v = {k: t[a : a + c] for k, t in v.items()}
+                elif name == "_system_extras":
+                    v = {k: t[s : s + 1] for k, t in v.items()}
                 elif name == "_rng":
                     v = None
                 kw[name] = v
@@ -211,6 +276,10 @@ def concatenate_states(states: list[SimState]) -> SimState:
             kw[name] = v0
         elif name == "_rng":
             kw[name] = None
+        elif name in SimState._extras_names:
+            # extras every state carries are concatenated; the others dropped
+            keys = [k for k in v0 if all(k in getattr(s, name) for s in states)]
+            kw[name] = {k: torch.cat([getattr(s, name)[k] for s in states]) for k in keys}
         elif isinstance(v0, torch.Tensor) and v0.ndim >= 1:
             kw[name] = torch.cat([getattr(s, name) for s in states])
         else:
@@ -246,6 +315,10 @@ def select_systems(state: SimState, sys_ids) -> SimState:
             kw[name] = v[atom_idx]
         elif isinstance(v, torch.Tensor) and name in state._system_attributes:
             kw[name] = v[ids]
+        elif name == "_atom_extras":
+            kw[name] = {k: t[atom_idx] for k, t in v.items()}
+        elif name == "_system_extras":
+            kw[name] = {k: t[ids] for k, t in v.items()}
         else:
             kw[name] = v
     return type(state)(**kw)
