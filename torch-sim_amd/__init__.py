@@ -63,6 +63,7 @@ from .runners import (
     optimize,
 )
 from .state import SimState, concatenate_states, initialize_state, select_systems
+from .trajectory import TorchSimTrajectory, TrajectoryReporter
 from .units import MetalUnits
 
 __version__ = "0.1.0"

@@ -6,7 +6,10 @@ of ``step_fn``.  When the model is a closed-form pair model, the integrator
 is NVE or Langevin NVT, the temperature is constant over the run and no
 reporter is attached, ``engine=None`` (default) routes the run through the
 fused, graph-captured :class:`~torch_sim_amd.engine.BatchedMD`; ``engine=False``
-keeps the reference's per-step loop.
+keeps the reference's per-step loop.  A trajectory reporter
+(runners.py:34-70, 108-250, 338-404) is called after every step of the
+per-step loop; on the engine the run is cut at the steps where the reporter
+writes something and the closed state of that step is reported.
 
 ``optimize`` (runners.py:567-753): ``init_fn`` on the systems in flight,
 ``steps_between_swaps`` steps, a per-system convergence check, removal of
@@ -32,9 +35,58 @@ from .integrators import INTEGRATOR_KWARG_UNITS, INTEGRATOR_REGISTRY, Integrator
 from .models import PairPotentialModel
 from .neighbors import SystemIndex, TensorKey, VerletList, assemble_verlet, segment_index
 from .optimizers import OPTIM_REGISTRY, Optimizer, fire_init, fire_step
-from .quantities import system_wise_max_force
+from .quantities import calc_kinetic_energy, system_wise_max_force
 from .state import SimState, concatenate_states, initialize_state, select_systems
+from .trajectory import TrajectoryReporter, _per_system
 from .units import MetalUnits
+
+
+def _configure_reporter(reporter, *, properties=None, prop_frequency: int = 10,
+                        state_frequency: int = 100) -> TrajectoryReporter:
+    """A reporter, or one built from a dict of its keywords with the named
+    default properties (runners.py:34-70)."""
+    if isinstance(reporter, TrajectoryReporter):
+        return reporter
+    possible = {
+        "potential_energy": _per_system(lambda state: state.energy),
+        "forces": _per_system(lambda state: state.forces),
+        "stress": _per_system(lambda state: state.stress),
+        "kinetic_energy": _per_system(lambda state: calc_kinetic_energy(
+            masses=state.masses, momenta=state.momenta, system_idx=state.system_idx)),
+        "temperature": _per_system(lambda state: state.calc_temperature()),
+        "max_force": _per_system(system_wise_max_force),
+    }
+    calcs = {k: fn for k, fn in possible.items() if k in (properties or ())}
+    kw = dict(reporter)
+    return TrajectoryReporter(
+        prop_calculators=kw.pop("prop_calculators", {prop_frequency: calcs}),
+        state_frequency=kw.pop("state_frequency", state_frequency),
+        **kw,
+    )
+
+
+def _initial_step(reporter) -> int:
+    """1, or one past the last step of appended files (runners.py:108-141)."""
+    if reporter is None or reporter.mode != "a":
+        return 1
+    last = [s if s is not None else 0 for s in reporter.last_steps]
+    if len(set(last)) != 1:
+        raise ValueError(
+            f"Trajectory files have different last steps: {set(last)} Cannot resume "
+            "integration from inconsistent states. You can truncate the trajectories to the "
+            "same step using:\n\n    reporter.truncate_to_step(min(reporter.last_step))\n\n"
+            "before calling integrate again.")
+    return 1 + min(last)
+
+
+def _write_initial_state(reporter, state, model, *, with_props: bool = True) -> None:
+    """Step 0 when every file is still empty (runners.py:228-250)."""
+    if reporter and all(t.last_step is None for t in reporter.trajectories):
+        reporter.report(state, 0, model=model, with_props=with_props)
+
+
+def _report_steps(reporter) -> list[int]:
+    return sorted({f for f in (reporter.state_frequency, *reporter.prop_calculators) if f})
 
 
 def _normalize_temperature_tensor(temperature, n_steps: int, state: SimState) -> torch.Tensor:
@@ -81,8 +133,6 @@ def integrate(
     **integrator_kwargs: Any,
 ):
     """Run MD (runners.py:252-413); see the module docstring for ``engine``."""
-    if trajectory_reporter is not None:
-        raise NotImplementedError("trajectory reporting is outside this package")
     if autobatcher:
         raise NotImplementedError("autobatching is outside this package")
     state = initialize_state(system, model.device, model.dtype)
@@ -119,7 +169,13 @@ def integrate(
     if use_engine and not engine_ok:
         raise ValueError("this model / integrator / schedule is not supported by the engine")
 
+    reporter = None
+    if trajectory_reporter is not None:
+        reporter = _configure_reporter(
+            trajectory_reporter, properties=["kinetic_energy", "potential_energy", "temperature"])
+    first = _initial_step(reporter)
     state = init_fn(state=state, model=model, kT=kTs[0], dt=dt, **init_kwargs)
+    _write_initial_state(reporter, state, model)
     if use_engine:
         md = BatchedMD(
             model,
@@ -132,10 +188,27 @@ def integrate(
             seed=int(torch.randint(0, 2**62, (1,)).item()) if seed is None else seed,
             tpa=getattr(model, "tpa", 4),
         )
-        md.run(n_steps)
-        return md.state()
-    for step in range(n_steps):
-        state = step_fn(state=state, model=model, dt=dt, kT=kTs[step], **integrator_kwargs)
+        if reporter is None:
+            md.run(n_steps)
+            return md.state()
+        freqs = _report_steps(reporter)
+        step, last = first, first + n_steps - 1
+        while step <= last:
+            k = min([((step + f - 1) // f) * f for f in freqs] + [last])
+            md.run(k - step + 1)
+            if any(k % f == 0 for f in freqs):
+                reporter.report(md.state(), k, model=model)
+            step = k + 1
+        out = md.state()
+        reporter.finish()
+        return out
+    for step in range(first, first + n_steps):
+        state = step_fn(state=state, model=model, dt=dt, kT=kTs[step - first],
+                        **integrator_kwargs)
+        if reporter:
+            reporter.report(state, step, model=model)
+    if reporter:
+        reporter.finish()
     return state
 
 
@@ -344,6 +417,8 @@ def _optimize_resident(state0, model, *, convergence_fn, max_steps, steps_betwee
         kept_atoms = int(counts[cur[keep_local]].sum()) if keep_local else 0
         new = take(budget - kept_atoms, at_least_one=not keep_local) if queue else []
         if not keep_local and not new:
+            if reporter is not None:
+                reporter.finish()
             break
         n_refills += 1
         state, fsi = assemble(state, fsi, keep_local, new, current)
@@ -380,8 +455,9 @@ def optimize(
     or models run the same schedule with host-side selection and
     concatenation.  ``stats`` (optional dict) receives the atom-steps, batch
     steps, swap checks and refills of the run."""
+    reporter = None
     if trajectory_reporter is not None:
-        raise NotImplementedError("trajectory reporting is outside this package")
+        reporter = _configure_reporter(trajectory_reporter, properties=["potential_energy"])
     if convergence_fn is None:
         convergence_fn = generate_energy_convergence_fn(energy_tol=1e-3)
     state0 = initialize_state(system, model.device, model.dtype)
@@ -405,7 +481,7 @@ def optimize(
         stats = {}
     for k in ("atom_steps", "batch_steps", "swaps", "refills"):
         stats.setdefault(k, 0)
-    if _resident_ok(model, init_fn, step_fn):
+    if reporter is None and _resident_ok(model, init_fn, step_fn):
         return _optimize_resident(state0, model, convergence_fn=convergence_fn,
                                   max_steps=max_steps, steps_between_swaps=steps_between_swaps,
                                   budget=budget, init_kwargs=init_kwargs,
@@ -413,6 +489,13 @@ def optimize(
     queue = list(range(n_total))
     done: dict[int, SimState] = {}
     steps = torch.ones(n_total, dtype=torch.int64)  # counted from 1 (runners.py:160-162)
+    og_files = None
+    if reporter is not None:
+        og_files = reporter.filenames
+        if reporter.mode == "a":  # resume after the files' last steps
+            steps += torch.tensor([s or 0 for s in reporter.last_steps], dtype=torch.int64)
+        # the input state at step 0 (it has no energies yet)
+        _write_initial_state(reporter, state0, model, with_props=False)
 
     def take(n_free_atoms: int, *, at_least_one: bool) -> list[int]:
         out, used = [], 0
@@ -426,12 +509,18 @@ def optimize(
 
     current = take(budget, at_least_one=True)
     state = init_fn(state=select_systems(state0, current), model=model, **init_kwargs)
+    files_of = None
     while True:
         exceeded = None
         last_energy = state.energy
+        if og_files is not None and files_of != current:
+            reporter.reopen_trajectories([og_files[i] for i in current])
+            files_of = list(current)
         for _ in range(steps_between_swaps):
             last_energy = state.energy
             state = step_fn(state=state, model=model, **optimizer_kwargs)
+            if reporter is not None:
+                reporter.report(state, steps[current].tolist(), model=model)
             stats["atom_steps"] += state.n_atoms
             stats["batch_steps"] += 1
             steps[current] += 1
