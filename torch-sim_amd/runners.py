@@ -417,8 +417,6 @@ def _optimize_resident(state0, model, *, convergence_fn, max_steps, steps_betwee
         kept_atoms = int(counts[cur[keep_local]].sum()) if keep_local else 0
         new = take(budget - kept_atoms, at_least_one=not keep_local) if queue else []
         if not keep_local and not new:
-            if reporter is not None:
-                reporter.finish()
             break
         n_refills += 1
         state, fsi = assemble(state, fsi, keep_local, new, current)
@@ -542,6 +540,8 @@ def optimize(
         kept_atoms = sum(counts[current[loc]] for loc in keep_local)
         new = take(budget - kept_atoms, at_least_one=not keep_local) if queue else []
         if not keep_local and not new:
+            if reporter is not None:
+                reporter.finish()
             break
         parts = []
         if keep_local:
