@@ -65,17 +65,11 @@ OVERHEAD_BYTES = 84.0
 # (bench.py --calibrate-copy 256: FETCH_SIZE x 2.000, WRITE_SIZE x 1.000).
 # Keyed by (workload, skin, tpa); "source" names the summary with the commands.
 MEASURED_TRAFFIC = {
-    ("cu_morse_nvt", 0.6, 2): {"fetch_kib": 111967.1, "fetch_factor": 2.000,
-                               "write_kib": 41957.1, "write_factor": 1.000,
-                               "source": "profiles/k_md_fused__cu_morse_nvt_512sys.md"},
-    ("cu_morse_nvt", 0.66, 2): {"fetch_kib": 117702.1, "fetch_factor": 2.000,
-                                "write_kib": 41769.9, "write_factor": 1.000,
+    ("cu_morse_nvt", 0.66, 2): {"fetch_kib": 119659.0, "fetch_factor": 2.000,
+                                "write_kib": 42546.3, "write_factor": 1.000,
                                 "source": "profiles/k_md_fused__cu_morse_nvt_512sys.md"},
-    ("cu_morse_nvt", 0.72, 2): {"fetch_kib": 123122.7, "fetch_factor": 2.000,
-                                "write_kib": 41832.4, "write_factor": 1.000,
-                                "source": "profiles/k_md_fused__cu_morse_nvt_512sys.md"},
-    ("lj1m_nve", 0.3, 4): {"fetch_kib": 342936.1, "fetch_factor": 2.000,
-                           "write_kib": 125831.8, "write_factor": 1.000,
+    ("lj1m_nve", 0.3, 4): {"fetch_kib": 324481.9, "fetch_factor": 2.000,
+                           "write_kib": 126536.7, "write_factor": 1.000,
                            "source": "profiles/k_md_fused__lj_fluid_1m_nve.md"},
 }
 
