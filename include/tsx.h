@@ -485,6 +485,129 @@ int tsx_npt_langevin_iso_post(int32_t n_systems, const int32_t* sys_start,
                               const double* ref_cell, const double* eps, double* eps_dot,
                               const double* cell_masses, const void* ws, void* stream);
 
+/* First half of one anisotropic NPT Langevin step (replaces
+ * torch_sim/integrators/npt.py:513-620, the cell and particle position
+ * updates of npt_langevin_anisotropic_step; state npt.py:62-131): the same
+ * GJF scheme with one strain eps_k per Cartesian axis ([S,3]), strain forces
+ * V (P_kk - p_ext[S,3]) with P_kk = -stress_kk + n kT / V and V = det(cell_col)
+ * of the current cell, the cell rows scaled by exp(eps_k') (cell_col updated
+ * in place), the particles' components by exp(eps_k' - eps_k).  cell_noise
+ * [S,3] and noise [N,3] are standard normals.  ws as tsx_npt_ws_bytes. */
+int tsx_npt_langevin_aniso_pre(int32_t n_systems, const int32_t* sys_start,
+                               const int32_t* atom_sys, int64_t n_atoms, double* pos,
+                               const double* mom, const double* forces, const double* masses,
+                               const double* kT, const double* p_ext, const double* alpha,
+                               const double* cell_alpha, const double* b_tau, double dt,
+                               const double* stress, const double* ref_cell, double* cell_col,
+                               double* eps, const double* eps_dot, double* cell_masses,
+                               const double* cell_noise, const double* noise, void* ws,
+                               void* stream);
+
+/* Second half (npt.py:622-650): per-axis strain velocities from the old and
+ * new strain forces (V of the new cell) and the GJF momentum update with the
+ * beta of tsx_npt_langevin_aniso_pre (same ws). */
+int tsx_npt_langevin_aniso_post(int32_t n_systems, const int32_t* sys_start,
+                                const int32_t* atom_sys, int64_t n_atoms, double* mom,
+                                const double* f_old, const double* f_new, const double* masses,
+                                const double* kT, const double* p_ext, const double* alpha,
+                                const double* cell_alpha, double dt, const double* stress,
+                                const double* cell_col, double* eps_dot,
+                                const double* cell_masses, const void* ws, void* stream);
+
+/* Workspace (bytes) of tsx_md_scale_kick_reduce and tsx_npt_nh_post: chunk
+ * partials of up to 6 doubles per (system, 2048-atom chunk). */
+int64_t tsx_mom_reduce_ws_bytes(int32_t max_system_atoms, int32_t n_systems);
+
+/* Per atom p <- scale[s] p + h F (scale and forces each optional, NULL =
+ * skip), then per system out[s] = factor * sum p^2 / m (tensor = 0), or the
+ * six components xx yy zz yz xz xy of sum p p^T / m ([S,6], tensor = 1), over
+ * the updated momenta; out NULL = no reduction.  Replaces the momentum
+ * rescales of torch_sim/integrators/md.py:488-493 and nvt.py:650-653, the
+ * momentum_step of md.py:169-187, calc_kinetic_energy (quantities.py:104-139)
+ * and the kinetic tensor of compute_instantaneous_pressure_tensor
+ * (quantities.py:156-198).  Sums run in a fixed order per system (2048-atom
+ * chunks from the system's first atom), independent of the batch. */
+int tsx_md_scale_kick_reduce(int32_t n_systems, const int32_t* sys_start,
+                             int32_t max_system_atoms, double* mom, const double* masses,
+                             const double* scale, const double* forces, double h,
+                             int32_t tensor, double factor, double* out, void* ws,
+                             void* stream);
+
+/* Per atom p <- scale[atom_sys] p + h F, x <- x + (p / m) dt: a per-system
+ * momentum scale fused with the first half of velocity Verlet
+ * (md.py:211-245 after nvt.py:455-459 / 650-653). scale NULL = 1. */
+int tsx_md_scale_kick_drift(int64_t n_atoms, const int32_t* atom_sys, double* pos, double* mom,
+                            const double* scale, const double* forces, double h,
+                            const double* masses, double dt, void* stream);
+
+/* Half a step of a Nose-Hoover chain per system (replaces the chain
+ * functions of torch_sim/integrators/md.py:338-565: half_step_chain_fn with
+ * substep_fn, and update_chain_mass_fn when update_mass): chain_steps x
+ * sy_steps Suzuki-Yoshida substeps over chain positions xi, momenta pxi and
+ * masses [S, chain_length]; ke [S] is the coupled kinetic energy (scaled in
+ * place), scale_out [S] the product of the momentum scales to apply to the
+ * particles, sys_mom [S] (optional) a per-system momentum scaled in place
+ * (the barostat's).  tau [S] is needed only with update_mass. */
+int tsx_nhc_half_step(int32_t n_systems, int32_t chain_length, int32_t chain_steps,
+                      int32_t sy_steps, double dt, const double* kT, const double* tau,
+                      const double* dof, double* xi, double* pxi, double* masses, double* ke,
+                      int32_t update_mass, double* scale_out, double* sys_mom, void* stream);
+
+/* CSVR momentum scale per system (replaces nvt.py:594-654
+ * _vrescale_update): ke2 = sum p^2 / m [S], dof, kT, tau [S]; r1 standard
+ * normal, r2 = 2 Gamma((dof - 1) / 2) draws [S]; lam [S] out. */
+int tsx_vrescale_factor(int32_t n_systems, const double* ke2, const double* dof,
+                        const double* kT, const double* tau, double dt, const double* r1,
+                        const double* r2, double* lam, void* stream);
+
+/* Workspace (bytes) of tsx_npt_nh_pre / _post: 8 doubles per system. */
+int64_t tsx_npt_nh_ws_bytes(int32_t n_systems);
+
+/* First half of the MTK inner step (replaces torch_sim/integrators/
+ * npt.py:1532-1590 and the cell mass of npt.py:1267-1309): per system the
+ * cell mass 3 (n + 1) kT b_tau^2, the cell force alpha 2K - tr(stress) V -
+ * 3 p_ext V (alpha = 1 + 3 / dof, K = ke2 / 2 of the thermostatted momenta,
+ * V = det(ref_cell) exp(3 eps)), cell_mom += dt/2 G, eps += (cell_mom / W)
+ * dt and cell_col = (V'/V0)^(1/3) ref_cell; per atom the momentum map
+ * exp(iL2 dt/2) with the current forces and the position map exp(iL1 dt). */
+int tsx_npt_nh_pre(int32_t n_systems, const int32_t* sys_start, const int32_t* atom_sys,
+                   int64_t n_atoms, double* pos, double* mom, const double* forces,
+                   const double* masses, const double* kT, const double* b_tau,
+                   const double* dof, const double* ke2, const double* stress,
+                   const double* p_ext, const double* ref_cell, double* eps, double* cell_mom,
+                   double* cell_mass, double* cell_col, double dt, void* ws, void* stream);
+
+/* Second half (npt.py:1591-1640, 1940-1960): exp(iL2 dt/2) with the new
+ * forces fused with ke2 = sum p^2 / m, the cell force at the new stress and
+ * volume, cell_mom += dt/2 G, ke_therm = ke2 / 2, ke_baro = cell_mom^2 / (2 W).
+ * ws from tsx_npt_nh_pre, red_ws as tsx_mom_reduce_ws_bytes. */
+int tsx_npt_nh_post(int32_t n_systems, const int32_t* sys_start, int32_t max_system_atoms,
+                    double* mom, const double* forces, const double* masses,
+                    const double* stress, const double* p_ext, const double* ref_cell,
+                    const double* eps, double* cell_mom, const double* cell_mass, double dt,
+                    double* ke2, double* ke_therm, double* ke_baro, const void* ws,
+                    void* red_ws, void* stream);
+
+/* Workspace (bytes) of tsx_crescale_step: 27 doubles per system. */
+int64_t tsx_crescale_ws_bytes(int32_t n_systems);
+
+/* One stochastic cell-rescaling barostat step (replaces torch_sim/
+ * integrators/npt.py:2186-2492): mode 0 isotropic, 1 anisotropic, 2
+ * triclinic (optional deviatoric target p_ext_tensor [S,3,3] with the
+ * initial cell inverse and volume), 3 triclinic with the average kinetic
+ * pressure (dof [S] = 3n - 3).  kin [S,6] = sum p p^T / m of the kicked
+ * momenta (modes 0-2).  noise [S,11] = (r_a, R[9] (mode 1: R[3]), r_b)
+ * standard normals.  Per system: sqrt(V) propagated in two halves around
+ * the traceless deformation (exp, Gram-Schmidt), the cell updated in place;
+ * per atom: x <- R x + (Mx p) dt / (2 m), p <- Vp p (unchanged in mode 3). */
+int tsx_crescale_step(int32_t n_systems, int32_t mode, const int32_t* atom_sys, int64_t n_atoms,
+                      double* pos, double* mom, const double* masses, const double* kin,
+                      const double* stress, double* cell_col, const double* kT,
+                      const double* compressibility, const double* tau_p, double dt,
+                      const double* p_ext, const double* p_ext_tensor, const double* cell0_inv,
+                      const double* vol0, const double* dof, const double* noise, void* ws,
+                      void* stream);
+
 /* One reverse step of the discrete adjoint of velocity Verlet in momentum
  * form (torch_sim/integrators/nve.py:74-130): xbar += vjp;
  * pbar += (dt / m) xbar; w = wscale * pbar (the next force-VJP cotangent). */

@@ -28,6 +28,9 @@ Modules:
     units   -- MetalUnits constants (torch_sim/units.py).
     rng     -- Philox4x32-10 + Box-Muller Langevin normals of the fused engine
                (the reference draws torch.randn, integrators/nvt.py:74-83).
+    thermo  -- Nose-Hoover chain, CSVR, anisotropic NPT Langevin, MTK
+               Nose-Hoover NPT and stochastic cell rescaling
+               (torch_sim/integrators/{md,nvt,npt}.py).
 """
 
 PINNING = {
@@ -89,4 +92,20 @@ PINNING = {
     "and the pi-digit pair)",
     "rng.langevin_normals": "Box-Muller identities (|z0, z1| radius from u1, angle from u2); "
     "moments over 2^18 draws",
+    "thermo.SY_WEIGHTS": "each set sums to 1 and is symmetric; float32 as the reference's "
+    "default-dtype tensors hold them",
+    "thermo.nhc_* / nvt_nose_hoover_step / nvt_nose_hoover_invariant": "harmonic "
+    "oscillators: the extended energy is conserved over 3000 steps and a cold start heats to "
+    "dof kT / 2; one substep and 3 x 3 Suzuki-Yoshida substeps agree to O(dt^3)",
+    "thermo.vrescale_update / nvt_vrescale_step": "ideal gas: the kinetic energy samples the "
+    "canonical gamma distribution (mean dof kT / 2, variance dof kT^2 / 2)",
+    "thermo.aniso_strain_force / npt_langevin_anisotropic_step": "the per-axis strain forces "
+    "sum to md.strain_force; at balanced pressure without noise or friction the strains move "
+    "ballistically and positions and cell rows scale per Cartesian axis",
+    "thermo.npt_nose_hoover_isotropic_step / _invariant": "soft-sphere crystal near its "
+    "virial pressure: the extended Hamiltonian is conserved over 300 steps",
+    "thermo.crescale_barostat / npt_crescale_step / rotate_gram_schmidt": "balanced pressure "
+    "without noise leaves the cell and drifts positions by p dt / m in every mode; the "
+    "triclinic deformation preserves volume (det(cell') = det(cell) (sqrt V' / sqrt V)^2); "
+    "Gram-Schmidt rows keep the columns' lengths and angles",
 }

@@ -10,19 +10,43 @@ kernels through the C ABI in ``include/tsx.h``.
 from . import _lib, neighbors, units, workloads
 from .engine import BatchedMD
 from .integrators import (
+    INTEGRATOR_KWARG_UNITS,
     INTEGRATOR_REGISTRY,
     Integrator,
     MDState,
+    NoseHooverChain,
+    NoseHooverChainFns,
+    NPTCRescaleState,
+    NPTLangevinAnisotropicState,
     NPTLangevinIsotropicState,
+    NPTNoseHooverIsotropicState,
     NPTState,
+    NVTNoseHooverState,
+    NVTVRescaleState,
+    construct_nose_hoover_chain,
     initialize_momenta,
+    momentum_step,
+    npt_crescale_anisotropic_step,
+    npt_crescale_init,
+    npt_crescale_isotropic_step,
+    npt_crescale_triclinic_average_step,
+    npt_crescale_triclinic_step,
+    npt_langevin_anisotropic_init,
+    npt_langevin_anisotropic_step,
     npt_langevin_isotropic_init,
     npt_langevin_isotropic_step,
-    momentum_step,
+    npt_nose_hoover_isotropic_init,
+    npt_nose_hoover_isotropic_invariant,
+    npt_nose_hoover_isotropic_step,
     nve_init,
     nve_step,
     nvt_langevin_init,
     nvt_langevin_step,
+    nvt_nose_hoover_init,
+    nvt_nose_hoover_invariant,
+    nvt_nose_hoover_step,
+    nvt_vrescale_init,
+    nvt_vrescale_step,
     position_step,
 )
 from .models import (

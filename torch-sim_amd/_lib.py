@@ -77,6 +77,33 @@ _SIGS = {
         I,
         [I, P, P, L, P, P, P, P, P, P, P, P, D, P, P, P, P, P, P, P],
     ),
+    "tsx_npt_langevin_aniso_pre": (
+        I,
+        [I, P, P, L, P, P, P, P, P, P, P, P, P, D, P, P, P, P, P, P, P, P, P, P],
+    ),
+    "tsx_npt_langevin_aniso_post": (
+        I,
+        [I, P, P, L, P, P, P, P, P, P, P, P, D, P, P, P, P, P, P],
+    ),
+    "tsx_mom_reduce_ws_bytes": (L, [I, I]),
+    "tsx_md_scale_kick_reduce": (I, [I, P, I, P, P, P, P, D, I, D, P, P, P]),
+    "tsx_md_scale_kick_drift": (I, [L, P, P, P, P, P, D, P, D, P]),
+    "tsx_nhc_half_step": (I, [I, I, I, I, D, P, P, P, P, P, P, P, I, P, P, P]),
+    "tsx_vrescale_factor": (I, [I, P, P, P, P, D, P, P, P, P]),
+    "tsx_npt_nh_ws_bytes": (L, [I]),
+    "tsx_npt_nh_pre": (
+        I,
+        [I, P, P, L, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, D, P, P],
+    ),
+    "tsx_npt_nh_post": (
+        I,
+        [I, P, I, P, P, P, P, P, P, P, P, P, D, P, P, P, P, P, P],
+    ),
+    "tsx_crescale_ws_bytes": (L, [I]),
+    "tsx_crescale_step": (
+        I,
+        [I, I, P, L, P, P, P, P, P, P, P, P, P, D, P, P, P, P, P, P, P, P],
+    ),
     "tsx_fire_ws_bytes": (L, [I]),
     "tsx_fire_pre": (
         I,

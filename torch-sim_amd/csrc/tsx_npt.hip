@@ -1,8 +1,10 @@
-// Isotropic NPT Langevin (gfx950): the Gronbech-Jensen-Farago integrator on
-// the particles and on the logarithmic strain eps = ln(V / V0) / 3 of each
-// system's cell (torch_sim/integrators/npt.py:653-1107).
+// NPT Langevin (gfx950): the Gronbech-Jensen-Farago integrator on the
+// particles and on the logarithmic strain of each system's cell -- one strain
+// eps = ln(V / V0) / 3 (isotropic, torch_sim/integrators/npt.py:653-1107) or
+// one per Cartesian axis (anisotropic, npt.py:62-650).
 //
-// A step is two launches around the model call:
+// A step is two launches around the model call (the anisotropic pair
+// tsx_npt_langevin_aniso_pre / _post does the same per axis):
 //   tsx_npt_langevin_iso_pre   per system: barostat mass Q = (n + 1) kT b_tau^2,
 //                              strain force F_eps = 3 V (P - P_ext) with
 //                              P = -tr(sigma) / 3 + n kT / V, the strain's GJF
@@ -18,7 +20,9 @@
 
 namespace tsx {
 
-constexpr int NPT_SW = 4;  // per-system scratch: F_eps(n), beta_c, eps(n), exp(eps' - eps)
+// per-system scratch: isotropic F_eps(n), beta_c, eps(n), exp(eps' - eps);
+// anisotropic F_eps(n)[3], beta_c[3], exp(eps' - eps)[3]
+constexpr int NPT_SW = 9;
 
 __global__ void k_npt_iso_sys_pre(int n_sys, const int32_t* __restrict__ sys_start,
                                   const double* __restrict__ kT, const double* __restrict__ p_ext,
@@ -132,6 +136,107 @@ __global__ void k_npt_iso_atoms_post(int64_t n_atoms, const int32_t* __restrict_
   }
 }
 
+// Anisotropic variant (npt.py:62-131, 409-650): one strain eps_k = ln(L_k /
+// L_k0) per Cartesian axis, driven by P_kk.  Per system: Q as above, the
+// strain forces F_k = V (P_kk - P_ext,k) with P_kk = -sigma_kk + n kT / V
+// (V = det of the current cell), GJF strain positions, the cell rows scaled
+// by exp(eps_k) (row k of a column-vector cell holds the k-th Cartesian
+// component of every lattice vector).
+__global__ void k_npt_aniso_sys_pre(int n_sys, const int32_t* __restrict__ sys_start,
+                                    const double* __restrict__ kT,
+                                    const double* __restrict__ p_ext,
+                                    const double* __restrict__ cell_alpha,
+                                    const double* __restrict__ b_tau, double dt,
+                                    const double* __restrict__ stress,
+                                    const double* __restrict__ ref_cell,
+                                    double* __restrict__ cell_col, double* __restrict__ eps,
+                                    const double* __restrict__ eps_dot,
+                                    double* __restrict__ cell_masses,
+                                    const double* __restrict__ cell_noise,
+                                    double* __restrict__ sw) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_sys) return;
+  const double n = (double)(sys_start[s + 1] - sys_start[s]);
+  const double kt = kT[s];
+  const double q = (n + 1.0) * kt * (b_tau[s] * b_tau[s]);
+  cell_masses[s] = q;
+  const double vol = det33(cell_col + 9 * (int64_t)s);
+  const double* sg = stress + 9 * (int64_t)s;
+  const double ca = cell_alpha[s];
+  const double q2 = 2.0 * q;
+  const double b = 1.0 / (1.0 + (ca * dt) / q2);
+  const double pre = sqrt(2.0 * ca * kt * dt);
+  const double* h0 = ref_cell + 9 * (int64_t)s;
+  double* sws = sw + NPT_SW * (int64_t)s;
+  for (int k = 0; k < 3; ++k) {
+    const double f_n = vol * ((-sg[4 * k] + n * kt / vol) - p_ext[3 * s + k]);
+    const double beta_c = pre * cell_noise[3 * s + k];
+    const double e0 = eps[3 * s + k];
+    const double e1 =
+        ((e0 + b * dt * eps_dot[3 * s + k]) + b * dt * dt * f_n / q2) + b * dt * beta_c / q2;
+    eps[3 * s + k] = e1;
+    const double sc = exp(e1);
+    for (int c = 0; c < 3; ++c) cell_col[9 * (int64_t)s + 3 * k + c] = sc * h0[3 * k + c];
+    sws[k] = f_n;
+    sws[3 + k] = beta_c;
+    sws[6 + k] = exp(e1 - e0);
+  }
+}
+
+__global__ void k_npt_aniso_atoms_pre(int64_t n_atoms, const int32_t* __restrict__ atom_sys,
+                                      double* __restrict__ pos, const double* __restrict__ mom,
+                                      const double* __restrict__ forces,
+                                      const double* __restrict__ masses,
+                                      const double* __restrict__ kT,
+                                      const double* __restrict__ alpha, double dt,
+                                      const double* __restrict__ noise,
+                                      const double* __restrict__ sw, double* __restrict__ beta) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_atoms) return;
+  const int s = atom_sys[i];
+  const double al = alpha[s];
+  const double mi = masses[i];
+  const double pre = sqrt(2.0 * al * kT[s] * dt);
+  const double m2 = 2.0 * mi;
+  const double b = 1.0 / (1.0 + (al * dt) / (2.0 * mi));
+  for (int c = 0; c < 3; ++c) {
+    const int64_t k = 3 * i + c;
+    const double sc = sw[NPT_SW * s + 6 + c];
+    const double bt = pre * noise[k];
+    beta[k] = bt;
+    const double v = mom[k] / mi;
+    const double c3 = (v + dt * forces[k] / m2) + bt / m2;
+    pos[k] = sc * pos[k] + ((2.0 * sc / (sc + 1.0)) * b * dt) * c3;
+  }
+}
+
+__global__ void k_npt_aniso_sys_post(int n_sys, const int32_t* __restrict__ sys_start,
+                                     const double* __restrict__ kT,
+                                     const double* __restrict__ p_ext,
+                                     const double* __restrict__ cell_alpha, double dt,
+                                     const double* __restrict__ stress,
+                                     const double* __restrict__ cell_col,
+                                     double* __restrict__ eps_dot,
+                                     const double* __restrict__ cell_masses,
+                                     const double* __restrict__ sw) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_sys) return;
+  const double n = (double)(sys_start[s + 1] - sys_start[s]);
+  const double kt = kT[s];
+  const double vol = det33(cell_col + 9 * (int64_t)s);
+  const double* sg = stress + 9 * (int64_t)s;
+  const double q = cell_masses[s];
+  const double ca = cell_alpha[s];
+  const double a = (1.0 - (ca * dt) / (2.0 * q)) / (1.0 + (ca * dt) / (2.0 * q));
+  const double b = 1.0 / (1.0 + (ca * dt) / (2.0 * q));
+  const double* sws = sw + NPT_SW * (int64_t)s;
+  for (int k = 0; k < 3; ++k) {
+    const double f_new = vol * ((-sg[4 * k] + n * kt / vol) - p_ext[3 * s + k]);
+    eps_dot[3 * s + k] =
+        (a * eps_dot[3 * s + k] + dt * ((a * sws[k]) + f_new) / (2.0 * q)) + b * sws[3 + k] / q;
+  }
+}
+
 }  // namespace tsx
 
 using namespace tsx;
@@ -186,6 +291,57 @@ int tsx_npt_langevin_iso_post(int32_t n_systems, const int32_t* sys_start,
   k_npt_iso_sys_post<<<blocks_for(n_systems, 64), 64, 0, st>>>(
       n_systems, sys_start, kT, p_ext, cell_alpha, dt, stress, ref_cell, eps, eps_dot,
       cell_masses, sw);
+  TSX_CHECK_LAUNCH();
+  if (n_atoms == 0) return TSX_OK;
+  k_npt_iso_atoms_post<<<blocks_for(n_atoms, 256), 256, 0, st>>>(
+      n_atoms, atom_sys, mom, f_old, f_new, masses, alpha, dt, beta);
+  TSX_CHECK_LAUNCH();
+  return TSX_OK;
+}
+
+int tsx_npt_langevin_aniso_pre(int32_t n_systems, const int32_t* sys_start,
+                               const int32_t* atom_sys, int64_t n_atoms, double* pos,
+                               const double* mom, const double* forces, const double* masses,
+                               const double* kT, const double* p_ext, const double* alpha,
+                               const double* cell_alpha, const double* b_tau, double dt,
+                               const double* stress, const double* ref_cell, double* cell_col,
+                               double* eps, const double* eps_dot, double* cell_masses,
+                               const double* cell_noise, const double* noise, void* ws,
+                               void* stream) {
+  TSX_REQUIRE(n_systems > 0, "n_systems must be positive");
+  TSX_REQUIRE(ws && kT && p_ext && alpha && cell_alpha && b_tau && stress && ref_cell &&
+                  cell_col && eps && eps_dot && cell_masses && cell_noise,
+              "every per-system array and the workspace are required");
+  hipStream_t st = (hipStream_t)stream;
+  double* sw = (double*)ws;
+  double* beta = sw + NPT_SW * (int64_t)n_systems;
+  k_npt_aniso_sys_pre<<<blocks_for(n_systems, 64), 64, 0, st>>>(
+      n_systems, sys_start, kT, p_ext, cell_alpha, b_tau, dt, stress, ref_cell, cell_col, eps,
+      eps_dot, cell_masses, cell_noise, sw);
+  TSX_CHECK_LAUNCH();
+  if (n_atoms == 0) return TSX_OK;
+  TSX_REQUIRE(pos && mom && forces && masses && noise && atom_sys, "per-atom arrays required");
+  k_npt_aniso_atoms_pre<<<blocks_for(n_atoms, 256), 256, 0, st>>>(
+      n_atoms, atom_sys, pos, mom, forces, masses, kT, alpha, dt, noise, sw, beta);
+  TSX_CHECK_LAUNCH();
+  return TSX_OK;
+}
+
+int tsx_npt_langevin_aniso_post(int32_t n_systems, const int32_t* sys_start,
+                                const int32_t* atom_sys, int64_t n_atoms, double* mom,
+                                const double* f_old, const double* f_new, const double* masses,
+                                const double* kT, const double* p_ext, const double* alpha,
+                                const double* cell_alpha, double dt, const double* stress,
+                                const double* cell_col, double* eps_dot,
+                                const double* cell_masses, const void* ws, void* stream) {
+  TSX_REQUIRE(n_systems > 0, "n_systems must be positive");
+  TSX_REQUIRE(ws && stress && cell_col && eps_dot, "workspace, stress, cell and eps_dot required");
+  hipStream_t st = (hipStream_t)stream;
+  const double* sw = (const double*)ws;
+  const double* beta = sw + NPT_SW * (int64_t)n_systems;
+  k_npt_aniso_sys_post<<<blocks_for(n_systems, 64), 64, 0, st>>>(
+      n_systems, sys_start, kT, p_ext, cell_alpha, dt, stress, cell_col, eps_dot, cell_masses,
+      sw);
   TSX_CHECK_LAUNCH();
   if (n_atoms == 0) return TSX_OK;
   k_npt_iso_atoms_post<<<blocks_for(n_atoms, 256), 256, 0, st>>>(

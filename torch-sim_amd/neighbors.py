@@ -118,6 +118,17 @@ class SystemIndex:
         return self._counts
 
 
+def state_system_index(state) -> SystemIndex:
+    """The SystemIndex of a state, kept on the state and rebuilt when its
+    ``system_idx`` tensor changes (the layout is validated once)."""
+    si = getattr(state, "_tsx_sysidx", None)
+    if si is None or not si.matches(state.system_idx, state.n_systems):
+        si = SystemIndex(state.system_idx, state.n_systems)
+        si.counts  # validates the layout once
+        object.__setattr__(state, "_tsx_sysidx", si)
+    return si
+
+
 def jbits_for(max_atoms_per_system: int) -> int:
     """Bits for a system-local atom index."""
     return max(1, int(math.ceil(math.log2(max(2, max_atoms_per_system)))))

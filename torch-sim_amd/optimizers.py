@@ -18,7 +18,7 @@ from typing import Any
 import torch
 
 from . import _lib
-from .neighbors import SystemIndex
+from .neighbors import SystemIndex, state_system_index
 from .state import SimState
 
 FIRE_FLAVORS = ("vv_fire", "ase_fire")
@@ -131,12 +131,7 @@ def _check(state: SimState) -> None:
 
 
 def _sysidx(state: SimState) -> SystemIndex:
-    si = getattr(state, "_tsx_sysidx", None)
-    if si is None or not si.matches(state.system_idx, state.n_systems):
-        si = SystemIndex(state.system_idx, state.n_systems)
-        si.counts  # validates the layout once
-        object.__setattr__(state, "_tsx_sysidx", si)
-    return si
+    return state_system_index(state)
 
 
 def _post(state: FireState, prm: _lib.FireParams, *, init: bool, active=None,
