@@ -126,13 +126,17 @@ const char* tsx_last_error(void);
  * (one 256-thread workgroup per bin: dense lists, partial rebuilds of many
  * systems);
  * TSX_FILL_ATOM gives each atom a group of lanes reading its stencil columns
- * from L2 (sparse bins, full builds of large systems); TSX_FILL_AUTO picks
- * BIN for bins_per_rlist >= 2, kmax >= 64 and more than one system.  Both
- * write every row in the same order (stencil image, then atom index).
+ * from L2 (sparse bins, full builds of large systems); TSX_FILL_WAVE (stencil
+ * words only) gives each active bin one wave that tests every candidate of
+ * the bin's stencil against all of the bin's atoms at once (the MD engine's
+ * gated rebuilds); TSX_FILL_AUTO picks BIN for bins_per_rlist >= 2,
+ * kmax >= 64 and more than one system, WAVE in its place for stencil words.
+ * All write every row in the same order (stencil image, then atom index).
  */
 #define TSX_FILL_AUTO 0
 #define TSX_FILL_BIN 1
 #define TSX_FILL_ATOM 2
+#define TSX_FILL_WAVE 3
 
 /* ``word_mode``: TSX_WORDS_SHIFT writes the image shift S' in the three word
  * fields (the layout above, read by every consumer of the reference triple);

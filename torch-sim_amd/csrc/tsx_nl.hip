@@ -1460,6 +1460,249 @@ void k_fill_atom(
   count[t] = cnt;
 }
 
+// --------------------------------------------------------- fill, per wave
+// Per-bin fill for stencil words (the MD engine's lists) with one wave per
+// active bin and every centre atom of the bin on every lane: a lane takes a
+// candidate of the bin's stencil (image order, then bin order, 64 per trip),
+// reads it once from L2 and tests it against up to FW_C centre atoms; one
+// ballot per centre orders the matches.  The candidate fetch, image lookup
+// and word assembly are paid once per candidate instead of once per
+// (candidate, centre), and the four waves of a workgroup run independent
+// bins with no workgroup barrier.  Rows equal k_fill's entry for entry.
+// floor(a / b) for |a| < 2^20 and 1 <= b < 2^20 through one FP32 product:
+// (a + 1/2) / b lies in [q, q + 1) with a margin of 1 / (2b) >> 2^-24 at
+// either end, so the rounded product floors to q exactly (no integer divide)
+__device__ __forceinline__ int floordiv_small(int a, float rb) {
+  return (int)floorf(((float)a + 0.5f) * rb);
+}
+
+// a wave-uniform double, moved to scalar registers
+__device__ __forceinline__ double uniform_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)b);
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+constexpr int FW_T = 256;     // threads per workgroup
+constexpr int FW_W = FW_T / 64;
+constexpr int FW_IMG = 128;   // stencil images per batch (two per lane)
+constexpr int FW_CH = 1024;   // candidates per image-table chunk
+constexpr int FW_C = 4;       // centre atoms per pass over the candidates
+
+__global__ __launch_bounds__(FW_T)
+void k_fill_wave(
+    const int32_t* __restrict__ sys_start, int n_sys, const double* __restrict__ geom,
+    const int32_t* __restrict__ grid, const int32_t* __restrict__ meta, int bin_cap,
+    const int32_t* __restrict__ act, const int2* __restrict__ witem,
+    const int32_t* __restrict__ bin_start, const double* __restrict__ wr,
+    const uint32_t* __restrict__ cw, double rl2, int self_int, int jbits, int lt, int kmax,
+    int64_t ld, uint32_t* __restrict__ nbr, int32_t* __restrict__ count,
+    int32_t* __restrict__ status) {
+  __shared__ int32_t s_ibeg[FW_W][FW_IMG];
+  __shared__ int32_t s_ipre[FW_W][FW_IMG];
+  __shared__ double s_ics[FW_W][FW_IMG][3];
+  __shared__ uint32_t s_sw[FW_W][FW_IMG];
+  __shared__ uint8_t s_cimg[FW_W][FW_CH];
+  if (meta[0] > bin_cap) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const WordCtx wc = word_ctx(jbits);
+  const int64_t stride = ld << lt;
+  const int tmask = (1 << lt) - 1;
+  const uint64_t below = (1ull << lane) - 1ull;
+  const int total_items = act[n_sys];
+  const int gstride = gridDim.x * FW_W;
+  auto item_of = [&](int wi) -> int2 {
+    if (witem) return witem[wi];
+    // the grid-wide build writes no items: search the active-bin prefix
+    const int si = active_system(act, n_sys, wi);
+    return make_int2(si, grid[(size_t)GRID * si + 6] + (wi - act[si]));
+  };
+  // What a bin needs before its images: the system's grid record and first
+  // atom, the bin's atom range.  The next item's record is loaded while the
+  // current bin runs, so a bin starts with its image ranges' round trip.
+  struct Rec {
+    int s, b, c0, c1, base;
+    int gr[7];
+  };
+  auto load_rec = [&](int wi) {
+    Rec r{};
+    r.b = -1;
+    if (wi < total_items) {
+      const int2 it = item_of(wi);
+      r.s = it.x;
+      r.b = it.y;
+      if (it.y >= 0) {
+        const int32_t* gp = grid + (size_t)GRID * it.x;
+        for (int k = 0; k < 7; ++k) r.gr[k] = gp[k];
+        r.c0 = bin_start[it.y];
+        r.c1 = bin_start[it.y + 1];
+        r.base = sys_start[it.x];
+      }
+    }
+    return r;
+  };
+  // a candidate of the current image chunk: its record and its image's
+  // shift and word fields
+  struct Cand {
+    double4 wj;
+    uint32_t cwj;
+    int q;
+  };
+  auto cand_load = [&](int c, int nk, int k0) {
+    Cand r;
+    const int cc = c < nk ? c : nk - 1;
+    r.q = s_cimg[w][cc];
+    const int k = s_ibeg[w][r.q] + (k0 + cc - s_ipre[w][r.q]);
+    r.wj = reinterpret_cast<const double4*>(wr)[k];
+    r.cwj = cw[k];
+    return r;
+  };
+  int wi = xcd_block() * FW_W + w;
+  Rec nxt = load_rec(wi);
+  for (; wi < total_items; wi += gstride) {
+    const Rec cur = nxt;
+    nxt = load_rec(wi + gstride);
+    if (cur.b < 0) continue;  // a system the build refused (uniform over the wave)
+    const int c0 = cur.c0, c1 = cur.c1;
+    if (c0 == c1) continue;
+    const int* gr = cur.gr;
+    if (!stencil_fits(gr, wc.bias)) {  // stencil offsets wider than the word fields
+      if (lane == 0) atomicOr(status, TSX_STATUS_SHIFT_OVERFLOW);
+      continue;
+    }
+    const double* g = geom + (size_t)GEOM * cur.s;
+    const int off = gr[6];
+    const int wx = 2 * gr[3] + 1, wy = 2 * gr[4] + 1, wz = 2 * gr[5] + 1;
+    const int n_img = wx * wy * wz;
+    const int local = cur.b - off;
+    const int bz0 = local % gr[2], by0 = (local / gr[2]) % gr[1], bx0 = local / (gr[1] * gr[2]);
+    // reciprocals for the image decode and the lattice shifts
+    const float r_wz = 1.0f / (float)wz, r_wyz = 1.0f / (float)(wy * wz);
+    const float r_nx = 1.0f / (float)gr[0], r_ny = 1.0f / (float)gr[1], r_nz = 1.0f / (float)gr[2];
+    for (int a0 = c0; a0 < c1; a0 += FW_C) {  // passes of up to FW_C centre atoms
+      const int na = min(FW_C, c1 - a0);
+      double cx[FW_C], cy[FW_C], cz[FW_C];
+      int il[FW_C], cnt[FW_C];
+#pragma unroll
+      for (int a = 0; a < FW_C; ++a) {
+        const int t = a0 + (a < na ? a : 0);
+        const double4 v = reinterpret_cast<const double4*>(wr)[t];
+        cx[a] = uniform_f64(v.x);
+        cy[a] = uniform_f64(v.y);
+        cz[a] = uniform_f64(v.z);
+        il[a] = __builtin_amdgcn_readfirstlane((int)(cw[t] & wc.jmask));
+        cnt[a] = 0;
+      }
+      for (int ib0 = 0; ib0 < n_img; ib0 += FW_IMG) {
+        // this lane's two images of the batch: bin range, shift, word fields
+        int n_q[2], kb[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int im = ib0 + lane + 64 * h;
+          n_q[h] = 0;
+          kb[h] = 0;
+          if (im < n_img) {
+            const int ix = floordiv_small(im, r_wyz);
+            const int iyz = im - ix * (wy * wz);
+            const int iy = floordiv_small(iyz, r_wz);
+            const int dz = iyz - iy * wz - gr[5];
+            const int dy = iy - gr[4];
+            const int dx = ix - gr[3];
+            const int qx = bx0 + dx, qy = by0 + dy, qz = bz0 + dz;
+            const int shx = floordiv_small(qx, r_nx), shy = floordiv_small(qy, r_ny),
+                      shz = floordiv_small(qz, r_nz);
+            const int nbin = off + ((qx - shx * gr[0]) * gr[1] + (qy - shy * gr[1])) * gr[2] +
+                             (qz - shz * gr[2]);
+            kb[h] = bin_start[nbin];
+            n_q[h] = bin_start[nbin + 1] - kb[h];
+            const int q = lane + 64 * h;
+            s_ics[w][q][0] = shift_comp(shx, shy, shz, g, 0);
+            s_ics[w][q][1] = shift_comp(shx, shy, shz, g, 1);
+            s_ics[w][q][2] = shift_comp(shx, shy, shz, g, 2);
+            s_sw[w][q] = pack_fields(dx, dy, dz, wc.sbits) << jbits;
+          }
+        }
+        // images in order: lanes' first images (0..63), then their second
+        const int inc0 = wave_incl_scan(n_q[0]);
+        const int inc1 = wave_incl_scan(n_q[1]);
+        const int tot0 = __shfl(inc0, 63, 64);
+        const int pre0 = inc0 - n_q[0], pre1 = tot0 + inc1 - n_q[1];
+        const int total = tot0 + __shfl(inc1, 63, 64);
+        s_ibeg[w][lane] = kb[0];
+        s_ibeg[w][lane + 64] = kb[1];
+        s_ipre[w][lane] = pre0;
+        s_ipre[w][lane + 64] = pre1;
+        for (int k0 = 0; k0 < total; k0 += FW_CH) {
+          const int nk = min(FW_CH, total - k0);
+          // candidate -> image table of this chunk, written by the image lanes
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int p0 = h ? pre1 : pre0;
+            const int e = min(p0 + n_q[h], k0 + nk);
+            for (int c = max(p0, k0); c < e; ++c) s_cimg[w][c - k0] = (uint8_t)(lane + 64 * h);
+          }
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          for (int cb = 0; cb < nk; cb += 64) {  // uniform trips over the wave
+            const Cand cd = cand_load(cb + lane, nk, k0);
+            const bool valid = cb + lane < nk;
+            const double sx = s_ics[w][cd.q][0], sy = s_ics[w][cd.q][1], sz = s_ics[w][cd.q][2];
+            const uint32_t swq = s_sw[w][cd.q];
+            const int jl = (int)(cd.cwj & wc.jmask);
+            const uint32_t word = (uint32_t)jl | swq;
+#pragma unroll
+            for (int a = 0; a < FW_C; ++a) {
+              if (a >= na) break;  // uniform over the wave
+              const double d2 = dist2_ref(cx[a], cy[a], cz[a], cd.wj.x, cd.wj.y, cd.wj.z, sx, sy, sz);
+              const bool match = valid && d2 < rl2 && (self_int || !(swq == 0 && jl == il[a]));
+              const uint64_t bal = __ballot(match);
+              if (match) {
+                const int pos = cnt[a] + __popcll(bal & below);
+                if (pos < kmax)
+                  nbr[((int64_t)(a0 + a) << lt) + (int64_t)(pos >> lt) * stride + (pos & tmask)] =
+                      word;
+              }
+              cnt[a] += __popcll(bal);
+            }
+          }
+          // every lane's reads of this chunk's table are done before it is rewritten
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < FW_C; ++a) {
+        if (a >= na) break;
+        if (lane == 0) {
+          int cn = cnt[a];
+          if (cn > kmax) {
+            atomicOr(status, TSX_STATUS_NL_OVERFLOW);
+            cn = kmax;
+          }
+          count[a0 + a] = cn;
+        }
+      }
+    }
+  }
+}
+
+static int fill_wave_resident_blocks() {
+  static int cache[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 1024;
+  if (cache[dev] > 0) return cache[dev];
+  int cus = 0, per = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_fill_wave, FW_T, 0) != hipSuccess ||
+      cus <= 0 || per <= 0)
+    return 1024;
+  cache[dev] = cus * per;
+  return cache[dev];
+}
+
 // Workgroups of the per-bin fill that fit on the device at once: its grid is
 // persistent, so a larger grid only adds a second, serial round of
 // workgroups.  Queried once per device and HALF variant.
@@ -1705,7 +1948,7 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
   TSX_REQUIRE(r_list > 0.0, "r_list must be positive");
   TSX_REQUIRE(bins_per_rlist >= 1 && bins_per_rlist <= 4, "bins_per_rlist in [1,4]");
   TSX_REQUIRE(bin_cap >= 1, "bin_cap must be >= 1");
-  TSX_REQUIRE(fill_mode >= TSX_FILL_AUTO && fill_mode <= TSX_FILL_ATOM, "unknown fill_mode");
+  TSX_REQUIRE(fill_mode >= TSX_FILL_AUTO && fill_mode <= TSX_FILL_WAVE, "unknown fill_mode");
   TSX_REQUIRE(word_mode == TSX_WORDS_SHIFT || word_mode == TSX_WORDS_STENCIL,
               "unknown word_mode");
   TSX_REQUIRE(word_mode == TSX_WORDS_SHIFT || (!half && jbits <= 22),
@@ -1730,6 +1973,11 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
   bool per_bin = fill_mode == TSX_FILL_BIN;
   if (fill_mode == TSX_FILL_AUTO) per_bin = bins_per_rlist >= 2 && kmax >= 64 && n_systems > 1;
   const bool small = max_system_atoms > 0 && max_system_atoms <= SB_MAXN;
+  // stencil words: one wave per active bin where the per-bin fill would run
+  const bool per_wave = fill_mode == TSX_FILL_WAVE ||
+                        (fill_mode == TSX_FILL_AUTO && per_bin && word_mode == TSX_WORDS_STENCIL);
+  TSX_REQUIRE(!per_wave || word_mode == TSX_WORDS_STENCIL, "TSX_FILL_WAVE needs stencil words");
+  if (per_wave) per_bin = true;  // the per-bin work items
   const bool red = frame && frame->e_atom;
   TSX_REQUIRE(!red || small, "the build's per-system reduction needs max_system_atoms in (0, 2048]");
   TSX_REQUIRE(!red || (frame->energy && frame->gate_clear && frame->step),
@@ -1742,8 +1990,9 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
         pos, pos_stride, sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, w.bin_count,
         w.atom_bin, w.bin_start, order, w.wr, gate, xref, counter, cell_col, cell_build, jbits,
         w.cw, w.wmax, word_mode, frame ? frame->anchor : nullptr, frame ? frame->rel : nullptr,
-        status, per_bin ? w.act : nullptr, per_bin ? w.witem : nullptr, red ? frame->e_atom : nullptr,
-        red ? frame->energy : nullptr, red ? frame->gate_clear : nullptr, red ? frame->step : nullptr);
+        status, per_bin ? w.act : nullptr, per_bin ? w.witem : nullptr,
+        red ? frame->e_atom : nullptr, red ? frame->energy : nullptr,
+        red ? frame->gate_clear : nullptr, red ? frame->step : nullptr);
     TSX_CHECK_LAUNCH();
   } else {
     // blocks per system: whole-grid parallelism for one large system, a few
@@ -1792,6 +2041,16 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
     }
     // stencil words come with full lists only (checked above)
     const bool sten = word_mode == TSX_WORDS_STENCIL;
+    if (per_wave) {
+      const int g_res_w = fill_wave_resident_blocks();
+      const int g_w = (bin_cap + FW_W - 1) / FW_W < g_res_w ? (bin_cap + FW_W - 1) / FW_W : g_res_w;
+      k_fill_wave<<<g_w, FW_T, 0, st>>>(sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap,
+                                        w.act, small ? w.witem : nullptr, w.bin_start, w.wr, w.cw,
+                                        rl2, self_interaction, jbits, lt, kmax, ld, nbr, count,
+                                        status);
+      TSX_CHECK_LAUNCH();
+      return TSX_OK;
+    }
     const int g_res = half   ? fill_resident_blocks<true, false>()
                       : sten ? fill_resident_blocks<false, true>()
                              : fill_resident_blocks<false, false>();
