@@ -130,7 +130,8 @@ const char* tsx_last_error(void);
  * words only) gives each active bin one wave that tests every candidate of
  * the bin's stencil against all of the bin's atoms at once (the MD engine's
  * gated rebuilds); TSX_FILL_AUTO picks BIN for bins_per_rlist >= 2,
- * kmax >= 64 and more than one system, WAVE in its place for stencil words.
+ * kmax >= 64 and more than one system, and WAVE for stencil words with
+ * bins_per_rlist >= 2 and kmax >= 64 (one large system included).
  * All write every row in the same order (stencil image, then atom index).
  */
 #define TSX_FILL_AUTO 0

@@ -18,6 +18,7 @@ oracle:
   energy drift;
 * ``integrate`` against the oracle's BAOAB fed the Philox normals, which
   pins the K / ps / (1/ps) conversions and the engine hand-off;
+* ``initialize_momenta`` against the oracle fed the generator's normals;
 * a freed ``system_idx`` whose memory is handed to a new layout.
 
 Tolerances are those of the other GPU tests: forces 1e-5 of max |F|,
@@ -421,6 +422,33 @@ def test_calc_kT_and_kinetic_energy_match_oracle(dev):
     assert abs(whole.item() - (ke.sum() * 2 / (3 * st.n_atoms)).item()) < 1e-12 * whole.item()
     with pytest.raises(ValueError):
         tsa.calc_kT(masses=m, momenta=p, velocities=v)
+
+
+def test_initialize_momenta_matches_oracle(dev):
+    """Maxwell-Boltzmann momenta on the GPU against the oracle fed the same
+    normals (the generator's stream redrawn): per-system kT, the per-system
+    mean removed where a system has more than one atom (a one-atom system
+    keeps its draw), a zero total momentum per system."""
+    reps = [[2, 2, 2], [3, 2, 2], [2, 3, 3]]
+    st = tsa.workloads.cu_batch(3, device=dev, reps=reps, seed=35)
+    lone = tsa.SimState(positions=torch.zeros(1, 3, dtype=F64, device=dev),
+                        masses=torch.full((1,), 63.546, dtype=F64, device=dev),
+                        cell=st.cell[:1].clone(), pbc=st.pbc,
+                        atomic_numbers=torch.full((1,), 29, dtype=torch.long, device=dev),
+                        system_idx=torch.zeros(1, dtype=torch.long, device=dev))
+    st = tsa.concatenate_states([st, lone])
+    kT = torch.tensor([300.0, 800.0, 1500.0, 1000.0], dtype=F64, device=dev) \
+        * tsa.MetalUnits.temperature
+    g = torch.Generator(device=dev).manual_seed(1234)
+    p = tsa.initialize_momenta(st.positions, st.masses, st.system_idx, kT, generator=g)
+    g.manual_seed(1234)
+    xi = torch.randn(st.positions.shape, device=dev, dtype=F64, generator=g)
+    ref = omd.initialize_momenta(st.masses.cpu(), st.system_idx.cpu(), kT.cpu(), xi.cpu())
+    torch.testing.assert_close(p.cpu(), ref, rtol=1e-13, atol=1e-15)
+    tot = torch.zeros(4, 3, dtype=F64).index_add(0, st.system_idx.cpu(), p.cpu())
+    assert tot[:3].abs().max().item() < 1e-10
+    lone_p = (xi[-1] * torch.sqrt(st.masses[-1] * kT[3])).cpu()
+    torch.testing.assert_close(p[-1].cpu(), lone_p, rtol=1e-15, atol=0)
 
 
 # ------------------------------------------- caches against reused memory

@@ -1969,13 +1969,16 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
   // Fill kernel: per-bin LDS staging pays off for dense lists of many
   // systems, where gated rebuilds touch a few percent of the atoms and one
   // workgroup per bin shares the staged candidates among the bin's atoms; the
-  // per-atom kernel wins for sparse bins and for single large systems.
+  // per-atom kernel wins for sparse bins and for single large systems.  With
+  // stencil words the one-wave-per-bin fill takes every dense list, one large
+  // system included (the 1M-atom LJ box: 2.58e9 against 2.46e9 atom-steps/s
+  // with the per-atom fill).
   bool per_bin = fill_mode == TSX_FILL_BIN;
-  if (fill_mode == TSX_FILL_AUTO) per_bin = bins_per_rlist >= 2 && kmax >= 64 && n_systems > 1;
+  const bool dense = bins_per_rlist >= 2 && kmax >= 64;
+  if (fill_mode == TSX_FILL_AUTO) per_bin = dense && n_systems > 1;
   const bool small = max_system_atoms > 0 && max_system_atoms <= SB_MAXN;
-  // stencil words: one wave per active bin where the per-bin fill would run
   const bool per_wave = fill_mode == TSX_FILL_WAVE ||
-                        (fill_mode == TSX_FILL_AUTO && per_bin && word_mode == TSX_WORDS_STENCIL);
+                        (fill_mode == TSX_FILL_AUTO && dense && word_mode == TSX_WORDS_STENCIL);
   TSX_REQUIRE(!per_wave || word_mode == TSX_WORDS_STENCIL, "TSX_FILL_WAVE needs stencil words");
   if (per_wave) per_bin = true;  // the per-bin work items
   const bool red = frame && frame->e_atom;
