@@ -1490,6 +1490,10 @@ constexpr int FW_IMG = 128;   // stencil images per batch (two per lane)
 constexpr int FW_CH = 1024;   // candidates per image-table chunk
 constexpr int FW_C = 4;       // centre atoms per pass over the candidates
 
+// NARROW: row slots addressed by a 24-bit multiply from a per-centre row
+// pointer (ld tpa below 2^24 and every slot index of a row below 2^31);
+// otherwise by 64-bit arithmetic
+template <bool NARROW>
 __global__ __launch_bounds__(FW_T)
 void k_fill_wave(
     const int32_t* __restrict__ sys_start, int n_sys, const double* __restrict__ geom,
@@ -1509,7 +1513,6 @@ void k_fill_wave(
   const WordCtx wc = word_ctx(jbits);
   const int64_t stride = ld << lt;
   const int tmask = (1 << lt) - 1;
-  const uint64_t below = (1ull << lane) - 1ull;
   const int total_items = act[n_sys];
   const int gstride = gridDim.x * FW_W;
   auto item_of = [&](int wi) -> int2 {
@@ -1659,10 +1662,19 @@ void k_fill_wave(
               const bool match = valid && d2 < rl2 && (self_int || !(swq == 0 && jl == il[a]));
               const uint64_t bal = __ballot(match);
               if (match) {
-                const int pos = cnt[a] + __popcll(bal & below);
-                if (pos < kmax)
-                  nbr[((int64_t)(a0 + a) << lt) + (int64_t)(pos >> lt) * stride + (pos & tmask)] =
-                      word;
+                const int pos = cnt[a] + (int)__builtin_amdgcn_mbcnt_hi(
+                                             (uint32_t)(bal >> 32),
+                                             __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                if (pos < kmax) {
+                  if (NARROW) {
+                    uint32_t* rowp = nbr + ((int64_t)(a0 + a) << lt);  // wave-uniform
+                    rowp[__umul24((uint32_t)(pos >> lt), (uint32_t)stride) +
+                         (uint32_t)(pos & tmask)] = word;
+                  } else {
+                    nbr[((int64_t)(a0 + a) << lt) + (int64_t)(pos >> lt) * stride +
+                        (pos & tmask)] = word;
+                  }
+                }
               }
               cnt[a] += __popcll(bal);
             }
@@ -1696,7 +1708,7 @@ static int fill_wave_resident_blocks() {
   if (cache[dev] > 0) return cache[dev];
   int cus = 0, per = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_fill_wave, FW_T, 0) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_fill_wave<true>, FW_T, 0) != hipSuccess ||
       cus <= 0 || per <= 0)
     return 1024;
   cache[dev] = cus * per;
@@ -2047,10 +2059,18 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
     if (per_wave) {
       const int g_res_w = fill_wave_resident_blocks();
       const int g_w = (bin_cap + FW_W - 1) / FW_W < g_res_w ? (bin_cap + FW_W - 1) / FW_W : g_res_w;
-      k_fill_wave<<<g_w, FW_T, 0, st>>>(sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap,
-                                        w.act, small ? w.witem : nullptr, w.bin_start, w.wr, w.cw,
-                                        rl2, self_interaction, jbits, lt, kmax, ld, nbr, count,
-                                        status);
+      const int64_t stride_w = ld * tpa;
+      const bool narrow = stride_w < (1 << 24) && (int64_t)(kmax / tpa + 1) * stride_w < (1ll << 31);
+      if (narrow)
+        k_fill_wave<true><<<g_w, FW_T, 0, st>>>(
+            sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, w.act,
+            small ? w.witem : nullptr, w.bin_start, w.wr, w.cw, rl2, self_interaction, jbits, lt,
+            kmax, ld, nbr, count, status);
+      else
+        k_fill_wave<false><<<g_w, FW_T, 0, st>>>(
+            sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, w.act,
+            small ? w.witem : nullptr, w.bin_start, w.wr, w.cw, rl2, self_interaction, jbits, lt,
+            kmax, ld, nbr, count, status);
       TSX_CHECK_LAUNCH();
       return TSX_OK;
     }
