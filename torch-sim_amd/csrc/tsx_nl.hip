@@ -1469,9 +1469,14 @@ void k_fill_atom(
 // and word assembly are paid once per candidate instead of once per
 // (candidate, centre), and the four waves of a workgroup run independent
 // bins with no workgroup barrier.  Rows equal k_fill's entry for entry.
-// floor(a / b) for |a| < 2^20 and 1 <= b < 2^20 through one FP32 product:
-// (a + 1/2) / b lies in [q, q + 1) with a margin of 1 / (2b) >> 2^-24 at
-// either end, so the rounded product floors to q exactly (no integer divide)
+// floor(a / b) for |a| < 2^20 and 1 <= b < 2^20 through one FP32 product,
+// with rb the hardware reciprocal of b (within 1 ulp, relative error at most
+// 2^-23): x = (a + 1/2) / b = q + (r + 1/2) / b lies at least 1 / (2b) inside
+// [q, q + 1), and the product misses x by at most x (2^-23 + 2^-24), which is
+// below 1 / (2b) whenever |a| + 1/2 < 2^22 / 3, so it floors to q exactly
+// (no integer divide)
+__device__ __forceinline__ float rcp_small(int b) { return __builtin_amdgcn_rcpf((float)b); }
+
 __device__ __forceinline__ int floordiv_small(int a, float rb) {
   return (int)floorf(((float)a + 0.5f) * rb);
 }
@@ -1579,10 +1584,20 @@ void k_fill_wave(
     const int wx = 2 * gr[3] + 1, wy = 2 * gr[4] + 1, wz = 2 * gr[5] + 1;
     const int n_img = wx * wy * wz;
     const int local = cur.b - off;
-    const int bz0 = local % gr[2], by0 = (local / gr[2]) % gr[1], bx0 = local / (gr[1] * gr[2]);
-    // reciprocals for the image decode and the lattice shifts
-    const float r_wz = 1.0f / (float)wz, r_wyz = 1.0f / (float)(wy * wz);
-    const float r_nx = 1.0f / (float)gr[0], r_ny = 1.0f / (float)gr[1], r_nz = 1.0f / (float)gr[2];
+    // reciprocals for the bin and image decodes and the lattice shifts
+    const float r_wz = rcp_small(wz), r_wyz = rcp_small(wy * wz);
+    const float r_nx = rcp_small(gr[0]), r_ny = rcp_small(gr[1]), r_nz = rcp_small(gr[2]);
+    int bx0, by0, bz0;
+    if (gr[0] * gr[1] * gr[2] <= (1 << 20)) {  // wave-uniform
+      const int lxy = floordiv_small(local, r_nz);
+      bz0 = local - lxy * gr[2];
+      bx0 = floordiv_small(lxy, r_ny);
+      by0 = lxy - bx0 * gr[1];
+    } else {
+      bz0 = local % gr[2];
+      by0 = (local / gr[2]) % gr[1];
+      bx0 = local / (gr[1] * gr[2]);
+    }
     for (int a0 = c0; a0 < c1; a0 += FW_C) {  // passes of up to FW_C centre atoms
       const int na = min(FW_C, c1 - a0);
       double cx[FW_C], cy[FW_C], cz[FW_C];
@@ -1606,17 +1621,22 @@ void k_fill_wave(
           n_q[h] = 0;
           kb[h] = 0;
           if (im < n_img) {
+            // unsigned 24-bit products of the non-negative image and bin
+            // indices (all far below 2^24)
             const int ix = floordiv_small(im, r_wyz);
-            const int iyz = im - ix * (wy * wz);
+            const int iyz = im - (int)__umul24((uint32_t)ix, (uint32_t)(wy * wz));
             const int iy = floordiv_small(iyz, r_wz);
-            const int dz = iyz - iy * wz - gr[5];
+            const int dz = iyz - (int)__umul24((uint32_t)iy, (uint32_t)wz) - gr[5];
             const int dy = iy - gr[4];
             const int dx = ix - gr[3];
             const int qx = bx0 + dx, qy = by0 + dy, qz = bz0 + dz;
             const int shx = floordiv_small(qx, r_nx), shy = floordiv_small(qy, r_ny),
                       shz = floordiv_small(qz, r_nz);
-            const int nbin = off + ((qx - shx * gr[0]) * gr[1] + (qy - shy * gr[1])) * gr[2] +
-                             (qz - shz * gr[2]);
+            const uint32_t wx_ = (uint32_t)(qx - shx * gr[0]), wy_ = (uint32_t)(qy - shy * gr[1]),
+                           wz_ = (uint32_t)(qz - shz * gr[2]);
+            const int nbin = off + (int)(__umul24(__umul24(wx_, (uint32_t)gr[1]) + wy_,
+                                                  (uint32_t)gr[2]) +
+                                         wz_);
             kb[h] = bin_start[nbin];
             n_q[h] = bin_start[nbin + 1] - kb[h];
             const int q = lane + 64 * h;
