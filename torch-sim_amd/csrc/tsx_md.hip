@@ -160,31 +160,80 @@ __device__ __forceinline__ void add_pair(RowAcc& a, float e, float fs, float dx,
   }
 }
 
-// d . ustep for a stencil offset d, per component.  With ORTHO (every cell
-// orthorhombic: ustep rows have zero off-diagonal entries) one product per
-// component, rounded as the general FMA chain rounds it when those entries
-// are zero (an FMA with a zero factor returns its addend): bitwise the same.
+// d = b + s_x u_0 + s_y u_1 + s_z u_2 for a stencil offset s and b = rel_j -
+// rel_i, per component, as an FMA chain onto b.  With ORTHO (every cell
+// orthorhombic: ustep rows have zero off-diagonal entries) one FMA per
+// component: an FMA with a zero factor returns its addend, so the general
+// chain gives the same bits on such cells.
 template <bool ORTHO>
-__device__ __forceinline__ void stencil_disp(float fx, float fy, float fz, const float* uf,
-                                             float& sx, float& sy, float& sz) {
-#pragma clang fp contract(off)
+__device__ __forceinline__ void stencil_fold(float fx, float fy, float fz, const float* uf,
+                                             float bx, float by, float bz, float& dx, float& dy,
+                                             float& dz) {
   if (ORTHO) {
-    sx = fx * uf[0];
-    sy = fy * uf[4];
-    sz = fz * uf[8];
+    dx = fmaf(fx, uf[0], bx);
+    dy = fmaf(fy, uf[4], by);
+    dz = fmaf(fz, uf[8], bz);
   } else {
-    sx = fmaf(fz, uf[6], fmaf(fy, uf[3], fx * uf[0]));
-    sy = fmaf(fz, uf[7], fmaf(fy, uf[4], fx * uf[1]));
-    sz = fmaf(fz, uf[8], fmaf(fy, uf[5], fx * uf[2]));
+    dx = fmaf(fz, uf[6], fmaf(fy, uf[3], fmaf(fx, uf[0], bx)));
+    dy = fmaf(fz, uf[7], fmaf(fy, uf[4], fmaf(fx, uf[1], by)));
+    dz = fmaf(fz, uf[8], fmaf(fy, uf[5], fmaf(fx, uf[2], bz)));
+  }
+}
+
+// FP32 displacement and squared distance of one list entry from its word and
+// the neighbour's relative position
+template <bool ORTHO>
+__device__ __forceinline__ void rel_disp(uint32_t w, const float4& q, int jbits, int sbits,
+                                         const float* uf, float xri, float yri, float zri,
+                                         float& dx, float& dy, float& dz, float& d2) {
+  const float fx = (float)nl_shift(w, jbits, sbits, 0);
+  const float fy = (float)nl_shift(w, jbits, sbits, 1);
+  const float fz = (float)nl_shift(w, jbits, sbits, 2);
+  stencil_fold<ORTHO>(fx, fy, fz, uf, q.x - xri, q.y - yri, q.z - zri, dx, dy, dz);
+  d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+}
+
+// Raw buffer loads: a uniform 128-bit resource, a per-lane 32-bit byte
+// offset and a scalar byte offset, so a batch's word addresses cost no VALU
+// work (the eight entries of a lane differ by scalar multiples of the row
+// stride).  No range clamp (num_records at its maximum): every offset below
+// addresses an element of the array.  aux 2 sets the non-temporal bit.
+constexpr int BUF_FLAGS = 0x00020000;  // gfx9 dword 3: 32-bit data format
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)0xffffffff,
+                                           BUF_FLAGS);
+}
+
+template <bool NT>
+__device__ __forceinline__ uint32_t buf_word(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                             uint32_t soff) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, NT ? 2 : 0);
+}
+
+// the neighbour's relative position: x, y, z (and the species bits with MULTI)
+template <bool MULTI>
+__device__ __forceinline__ float4 buf_rel(__amdgpu_buffer_rsrc_t r, uint32_t voff) {
+  if (MULTI) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, 0, 0);
+    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                       __uint_as_float(v[3]));
+  } else {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, (int)voff, 0, 0);
+    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                       0.f);
   }
 }
 
 // NTW: load the list words non-temporally (a compile-time choice: as a
-// runtime branch the compiler merges the two loads and drops the hint)
+// runtime branch the compiler merges the two loads and drops the hint).
+// Byte offsets are 32-bit: the caller guarantees 4 * (rows * TPA) and
+// 4 * 7 * ld * TPA below 2^32 and 16 * N below 2^32.
 template <int KIND, int TPA, bool VIRIAL, bool MULTI, bool ORTHO, bool NTW>
 __device__ __forceinline__ RowAcc accumulate_rel(const PairP* sp, int ns, int ti,
-                                                 const float4* __restrict__ rel_s, float xri,
-                                                 float yri, float zri, const float* uf,
+                                                 const float* __restrict__ rel_all,
+                                                 uint32_t rel_base, float xri, float yri,
+                                                 float zri, const float* uf,
                                                  const uint32_t* __restrict__ nbr, int64_t t,
                                                  int cnt, int lane, int jbits, int64_t ld,
                                                  float lo2, float hi2, const ExactCtx& ex) {
@@ -194,42 +243,42 @@ __device__ __forceinline__ RowAcc accumulate_rel(const PairP* sp, int ns, int ti
   const int sbits = (32 - jbits) / 3;
   const uint32_t jmask = (1u << jbits) - 1u;
   const PairP p0 = sp[MULTI ? 0 : ti * ns];
-  const uint32_t* wp = nbr + t * TPA + lane;
   const int64_t stride = ld * TPA;
   const int iters = cnt > lane ? (cnt - lane + TPA - 1) / TPA : 0;
+  const __amdgpu_buffer_rsrc_t rrel = buf_rsrc(rel_all);
+  const uint32_t woff = (uint32_t)(t * TPA + lane) * 4u;  // entry 0 of this lane's share
+  const uint32_t sstep = (uint32_t)stride * 4u;           // bytes between a lane's entries
   // Branch-free batches: across a wave nearly every entry of a batch is in
   // range on some lane, so the pair arithmetic runs for all of them anyway;
   // per-entry branches would only add exec-mask traffic.  Only a lane's last
   // batch can run past its row's end: there the lane re-reads its last entry
   // (the load needs no guard) and the entries past the end add exact zeros,
   // so the sums keep the bits of an entry-by-entry walk.
-  const uint32_t* wlast = wp + (int64_t)(iters - 1) * stride;  // used when iters > 0
-  const uint32_t* pk = wp;  // entry k0 of this lane
   auto batch = [&](int k0, auto tail_tag) {
     constexpr bool TAIL = decltype(tail_tag)::value;
     uint32_t w[UNR_REL];
+    if (!TAIL) {
+      // the resource's base steps by whole batches (scalar), lanes keep woff
+      const __amdgpu_buffer_rsrc_t rw = buf_rsrc(nbr + (int64_t)k0 * stride);
 #pragma unroll
-    for (int u = 0; u < UNR_REL; ++u) {
-      const uint32_t* pa = (!TAIL || k0 + u < iters) ? pk : wlast;
-      w[u] = NTW ? __builtin_nontemporal_load(pa) : *pa;
-      pk += stride;
+      for (int u = 0; u < UNR_REL; ++u) w[u] = buf_word<NTW>(rw, woff, (uint32_t)u * sstep);
+    } else {
+      const __amdgpu_buffer_rsrc_t rw = buf_rsrc(nbr + (int64_t)k0 * stride);
+#pragma unroll
+      for (int u = 0; u < UNR_REL; ++u) {
+        const int k = k0 + u < iters ? u : iters - 1 - k0;  // past the end: the last entry
+        w[u] = buf_word<NTW>(rw, woff + (uint32_t)k * sstep, 0u);
+      }
     }
     float4 q[UNR_REL];
 #pragma unroll
-    for (int u = 0; u < UNR_REL; ++u) q[u] = rel_s[w[u] & jmask];
+    for (int u = 0; u < UNR_REL; ++u) q[u] = buf_rel<MULTI>(rrel, ((w[u] & jmask) + rel_base) << 4);
     uint32_t amb = 0;  // entries within the rounding margin of rc
 #pragma unroll
     for (int u = 0; u < UNR_REL; ++u) {
       const bool valid = !TAIL || k0 + u < iters;
-      const float fx = (float)nl_shift(w[u], jbits, sbits, 0);
-      const float fy = (float)nl_shift(w[u], jbits, sbits, 1);
-      const float fz = (float)nl_shift(w[u], jbits, sbits, 2);
-      float sx, sy, sz;
-      stencil_disp<ORTHO>(fx, fy, fz, uf, sx, sy, sz);
-      const float dx = (q[u].x - xri) + sx;
-      const float dy = (q[u].y - yri) + sy;
-      const float dz = (q[u].z - zri) + sz;
-      const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+      float dx, dy, dz, d2;
+      rel_disp<ORTHO>(w[u], q[u], jbits, sbits, uf, xri, yri, zri, dx, dy, dz, d2);
       const bool in = valid && d2 < lo2;
       amb |= (valid && !(d2 < lo2) && d2 < hi2) ? (1u << u) : 0u;
       float e, fs;
@@ -322,11 +371,10 @@ void k_md_fused(FusedArgs A) {
   ex.anc_s = reinterpret_cast<const double4*>(A.anchor) + base;
   ex.u = us;
   ex.rc2 = A.rc * A.rc;
-  const float4* rel_s = reinterpret_cast<const float4*>(A.rel_in) + base;
   const int cnt = valid ? A.count[tt] : 0;
   const RowAcc a = accumulate_rel<KIND, TPA, VIRIAL, MULTI, ORTHO, NTW>(
-      sp, A.ns, ti, rel_s, (float)ex.xi, (float)ex.yi, (float)ex.zi, uf, A.nbr, tt, cnt, lane,
-      A.jbits, A.ld, lo2, hi2, ex);
+      sp, A.ns, ti, A.rel_in, (uint32_t)base, (float)ex.xi, (float)ex.yi, (float)ex.zi, uf,
+      A.nbr, tt, cnt, lane, A.jbits, A.ld, lo2, hi2, ex);
   // Philox normals of (seed, step, atom) for the O step: with two or more
   // lanes per atom, lanes 0 and 1 each draw one of normal3's two blocks (the
   // same operations on each block, so the same values), in one instruction
@@ -523,6 +571,12 @@ int tsx_md_fused_step(int32_t kind, const double* params, int32_t n_species,
   TSX_REQUIRE(rel_in && rel_out && rel_in != rel_out && anchor && ustep,
               "the fused step needs the list frame: rel_in, rel_out, anchor, ustep");
   TSX_REQUIRE(cutoff > 0.0 && half_skin >= 0.0, "need cutoff > 0 and half_skin >= 0");
+  TSX_REQUIRE(ld >= n_atoms, "ld must be at least n_atoms");
+  // 32-bit byte offsets of the buffer loads: a batch spans 8 row strides of
+  // words, relative positions 16 bytes per atom
+  TSX_REQUIRE(ld * (int64_t)tpa * 4 * 8 < ((int64_t)1 << 32) && n_atoms * 16 < ((int64_t)1 << 32),
+              "list too large for one fused launch (ld * tpa = %lld): split the batch",
+              (long long)(ld * tpa));
   if (n_atoms == 0) return TSX_OK;
   FusedArgs a;
   a.params = params;

@@ -8,18 +8,21 @@ namespace tsx {
 
 // Derived per-species-pair constants in FP32:
 //   LJ:          a = sigma^2, b = 4 eps,  c = 24 eps
-//   Morse:       a = sigma,   b = eps,    c = alpha,  d = 2 eps alpha
+//   Morse:       a = sigma,   b = eps,    c = alpha,  d = 2 eps alpha,
+//                e = sigma alpha log2(e), f = -alpha log2(e)
+//                (exp(-alpha (r - sigma)) = exp2(f r + e): one FMA and v_exp)
 //   soft sphere: a = sigma,   b = eps,    c = alpha,  d = eps / sigma
 //   particle life (sigma, eps, alpha = outer radius s, amplitude A, inner
 //   radius b): a = b, b = A, c = s, d = 1 / b, e = 1 / (1 - b)
 struct PairP {
-  float a, b, c, d, e;
+  float a, b, c, d, e, f;
 };
 
 __host__ __device__ inline PairP make_pair_params(int kind, double sigma, double eps,
                                                   double alpha) {
   PairP p;
   p.e = (float)(1.0 / sigma);
+  p.f = 0.f;
   if (kind == TSX_PARTICLE_LIFE) {
     p.a = (float)alpha;
     p.b = (float)eps;
@@ -36,6 +39,8 @@ __host__ __device__ inline PairP make_pair_params(int kind, double sigma, double
     p.b = (float)eps;
     p.c = (float)alpha;
     p.d = (float)(2.0 * eps * alpha);
+    p.e = (float)(sigma * alpha * 1.4426950408889634);
+    p.f = (float)(-alpha * 1.4426950408889634);
   } else {
     p.a = (float)sigma;
     p.b = (float)eps;
@@ -71,7 +76,7 @@ __device__ __forceinline__ void pair_eval(float r2, const PairP& p, float& e, fl
       // one reciprocal square root gives r = r2 / sqrt(r2) and 1 / r
       const float ir = frsq(r2);
       const float r = r2 * ir;
-      const float x = __expf(-p.c * (r - p.a));
+      const float x = __builtin_amdgcn_exp2f(fmaf(p.f, r, p.e));
       const float omx = 1.f - x;
       e = p.b * omx * omx - p.b;
       fs = -p.d * x * omx * ir;
@@ -126,7 +131,7 @@ __device__ __forceinline__ void pair_eval_pos(float r2, const PairP& p, float& e
   } else if (KIND == TSX_MORSE) {
     const float ir = frsq(r2);
     const float r = r2 * ir;
-    const float x = __expf(-p.c * (r - p.a));
+    const float x = __builtin_amdgcn_exp2f(fmaf(p.f, r, p.e));
     const float omx = 1.f - x;
     e = p.b * omx * omx - p.b;
     fs = -p.d * x * omx * ir;
