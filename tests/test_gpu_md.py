@@ -264,10 +264,10 @@ def test_engine_batch_equals_single_bitwise_with_rebuilds(dev):
 
 def test_engine_fill_kernels_give_the_same_trajectory(dev):
     """Hot Cu (2000 K) with a small skin rebuilds many systems' lists; the
-    per-bin, per-atom and one-wave-per-bin fills write the same rows, so the
-    trajectories and energies are bitwise the same, and so are consecutive
-    runs against one run of their total length (the engine leaves the last
-    step open)."""
+    per-bin, per-atom and one-wave-per-bin fills (4 or 6 centres per pass)
+    write the same rows, so the trajectories and energies are bitwise the
+    same, and so are consecutive runs against one run of their total length
+    (the engine leaves the last step open)."""
     st = tsa.workloads.cu_batch(5, device=dev, seed=28)
     model = tsa.MorseModel(**MORSE, cutoff=6.0, device=dev)
     st.rng = 6
@@ -275,7 +275,7 @@ def test_engine_fill_kernels_give_the_same_trajectory(dev):
     md = tsa.nvt_langevin_init(state=st, model=model, kT=kT)
     dt = 0.002 * tsa.MetalUnits.time
     runs = []
-    for mode in (1, 2, 3):
+    for mode in (1, 2, 3, 4, 5):
         eng = tsa.BatchedMD(model, md, dt=dt, kT=kT, langevin=True, skin=0.2, seed=9)
         eng.nl.fill_mode = mode
         print("fill mode", mode, flush=True)

@@ -158,21 +158,22 @@ def test_fill_kernels_write_identical_rows(dev, q):
 @pytest.mark.parametrize("q", [1, 2, 3])
 def test_wave_fill_writes_the_same_stencil_rows(dev, q):
     """Stencil-word lists (the MD engine's): the one-wave-per-bin fill (mode
-    3) writes every row entry for entry as the per-bin and per-atom fills
-    do, on dense Cu systems, small cells with many periodic images and
-    triclinic soft-sphere cells."""
+    3, and modes 4 / 5 with 4 / 6 centre atoms per pass) writes every row
+    entry for entry as the per-bin and per-atom fills do, on dense Cu
+    systems (bins of up to 10 atoms: one and two passes of 6), small cells
+    with many periodic images and triclinic soft-sphere cells."""
     st = tsa.workloads.cu_batch(6, device="cpu", seed=8)
     small = tsa.workloads.cu_batch(3, device="cpu", reps=[[2, 2, 2], [1, 1, 1], [1, 2, 3]], seed=2)
     tri = tsa.workloads.soft_sphere_batch(4, n_min=40, n_max=120, device="cpu", seed=3)
     for s_ in (st, small, tri):
         si = SystemIndex(s_.system_idx.cuda(), s_.n_systems)
         outs = []
-        for mode in (1, 2, 3):
+        for mode in (1, 2, 3, 4, 5):
             print("stencil words, fill mode", mode, "q", q, "atoms", s_.n_atoms, flush=True)
             nl = NeighborList(6.0, tpa=2, bins_per_rlist=q, fill_mode=mode, word_mode=1)
             nl.build(s_.positions.cuda(), s_.cell.cuda().contiguous(), s_.pbc.cuda(), si)
             outs.append(_raw_rows(nl, s_.n_atoms))
-        assert outs[0] == outs[1] == outs[2]
+        assert all(o == outs[0] for o in outs[1:])
 
 
 @pytest.mark.parametrize("mode", [1, 2])

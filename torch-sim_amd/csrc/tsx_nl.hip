@@ -1464,7 +1464,7 @@ void k_fill_atom(
 // Per-bin fill for stencil words (the MD engine's lists) with one wave per
 // active bin and every centre atom of the bin on every lane: a lane takes a
 // candidate of the bin's stencil (image order, then bin order, 64 per trip),
-// reads it once from L2 and tests it against up to FW_C centre atoms; one
+// reads it once from L2 and tests it against up to NC centre atoms; one
 // ballot per centre orders the matches.  The candidate fetch, image lookup
 // and word assembly are paid once per candidate instead of once per
 // (candidate, centre), and the four waves of a workgroup run independent
@@ -1493,12 +1493,19 @@ constexpr int FW_T = 256;     // threads per workgroup
 constexpr int FW_W = FW_T / 64;
 constexpr int FW_IMG = 128;   // stencil images per batch (two per lane)
 constexpr int FW_CH = 1024;   // candidates per image-table chunk
-constexpr int FW_C = 4;       // centre atoms per pass over the candidates
+// Centre atoms per pass over the candidates (NC): a bin with more atoms walks
+// its stencil's candidates again.  4 suits sparse bins; 6 suits dense ones,
+// where fcc atoms on bin faces and corners make a third of the bins hold 5
+// or more (one unit cell per bin: 4 atoms on average, up to 10), at 123
+// instead of 117 VGPRs (still four waves per SIMD).
+constexpr int FW_C_SPARSE = 4;
+constexpr int FW_C_DENSE = 6;
+constexpr double FW_DENSE_OCC = 3.0;  // mean atoms per bin from which NC is 6
 
 // NARROW: row slots addressed by a 24-bit multiply from a per-centre row
 // pointer (ld tpa below 2^24 and every slot index of a row below 2^31);
 // otherwise by 64-bit arithmetic
-template <bool NARROW>
+template <bool NARROW, int NC>
 __global__ __launch_bounds__(FW_T)
 void k_fill_wave(
     const int32_t* __restrict__ sys_start, int n_sys, const double* __restrict__ geom,
@@ -1598,12 +1605,12 @@ void k_fill_wave(
       by0 = (local / gr[2]) % gr[1];
       bx0 = local / (gr[1] * gr[2]);
     }
-    for (int a0 = c0; a0 < c1; a0 += FW_C) {  // passes of up to FW_C centre atoms
-      const int na = min(FW_C, c1 - a0);
-      double cx[FW_C], cy[FW_C], cz[FW_C];
-      int il[FW_C], cnt[FW_C];
+    for (int a0 = c0; a0 < c1; a0 += NC) {  // passes of up to NC centre atoms
+      const int na = min(NC, c1 - a0);
+      double cx[NC], cy[NC], cz[NC];
+      int il[NC], cnt[NC];
 #pragma unroll
-      for (int a = 0; a < FW_C; ++a) {
+      for (int a = 0; a < NC; ++a) {
         const int t = a0 + (a < na ? a : 0);
         const double4 v = reinterpret_cast<const double4*>(wr)[t];
         cx[a] = uniform_f64(v.x);
@@ -1676,7 +1683,7 @@ void k_fill_wave(
             const int jl = (int)(cd.cwj & wc.jmask);
             const uint32_t word = (uint32_t)jl | swq;
 #pragma unroll
-            for (int a = 0; a < FW_C; ++a) {
+            for (int a = 0; a < NC; ++a) {
               if (a >= na) break;  // uniform over the wave
               const double d2 = dist2_ref(cx[a], cy[a], cz[a], cd.wj.x, cd.wj.y, cd.wj.z, sx, sy, sz);
               const bool match = valid && d2 < rl2 && (self_int || !(swq == 0 && jl == il[a]));
@@ -1706,7 +1713,7 @@ void k_fill_wave(
         }
       }
 #pragma unroll
-      for (int a = 0; a < FW_C; ++a) {
+      for (int a = 0; a < NC; ++a) {
         if (a >= na) break;
         if (lane == 0) {
           int cn = cnt[a];
@@ -1721,6 +1728,7 @@ void k_fill_wave(
   }
 }
 
+template <int NC>
 static int fill_wave_resident_blocks() {
   static int cache[64] = {0};
   int dev = 0;
@@ -1728,7 +1736,7 @@ static int fill_wave_resident_blocks() {
   if (cache[dev] > 0) return cache[dev];
   int cus = 0, per = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_fill_wave<true>, FW_T, 0) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_fill_wave<true, NC>, FW_T, 0) != hipSuccess ||
       cus <= 0 || per <= 0)
     return 1024;
   cache[dev] = cus * per;
@@ -1980,7 +1988,7 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
   TSX_REQUIRE(r_list > 0.0, "r_list must be positive");
   TSX_REQUIRE(bins_per_rlist >= 1 && bins_per_rlist <= 4, "bins_per_rlist in [1,4]");
   TSX_REQUIRE(bin_cap >= 1, "bin_cap must be >= 1");
-  TSX_REQUIRE(fill_mode >= TSX_FILL_AUTO && fill_mode <= TSX_FILL_WAVE, "unknown fill_mode");
+  TSX_REQUIRE(fill_mode >= TSX_FILL_AUTO && fill_mode <= TSX_FILL_WAVE6, "unknown fill_mode");
   TSX_REQUIRE(word_mode == TSX_WORDS_SHIFT || word_mode == TSX_WORDS_STENCIL,
               "unknown word_mode");
   TSX_REQUIRE(word_mode == TSX_WORDS_SHIFT || (!half && jbits <= 22),
@@ -2009,7 +2017,7 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
   const bool dense = bins_per_rlist >= 2 && kmax >= 64;
   if (fill_mode == TSX_FILL_AUTO) per_bin = dense && n_systems > 1;
   const bool small = max_system_atoms > 0 && max_system_atoms <= SB_MAXN;
-  const bool per_wave = fill_mode == TSX_FILL_WAVE ||
+  const bool per_wave = fill_mode >= TSX_FILL_WAVE ||
                         (fill_mode == TSX_FILL_AUTO && dense && word_mode == TSX_WORDS_STENCIL);
   TSX_REQUIRE(!per_wave || word_mode == TSX_WORDS_STENCIL, "TSX_FILL_WAVE needs stencil words");
   if (per_wave) per_bin = true;  // the per-bin work items
@@ -2077,20 +2085,29 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
     // stencil words come with full lists only (checked above)
     const bool sten = word_mode == TSX_WORDS_STENCIL;
     if (per_wave) {
-      const int g_res_w = fill_wave_resident_blocks();
+      // centres per pass from the mean bin occupancy (bin_cap is the caller's
+      // bin count plus a margin) unless the mode names it
+      const bool dense_bins = fill_mode == TSX_FILL_WAVE6 ||
+                              (fill_mode != TSX_FILL_WAVE4 &&
+                               (double)n_atoms >= FW_DENSE_OCC * (double)bin_cap);
+      const int g_res_w = dense_bins ? fill_wave_resident_blocks<FW_C_DENSE>()
+                                     : fill_wave_resident_blocks<FW_C_SPARSE>();
       const int g_w = (bin_cap + FW_W - 1) / FW_W < g_res_w ? (bin_cap + FW_W - 1) / FW_W : g_res_w;
       const int64_t stride_w = ld * tpa;
       const bool narrow = stride_w < (1 << 24) && (int64_t)(kmax / tpa + 1) * stride_w < (1ll << 31);
-      if (narrow)
-        k_fill_wave<true><<<g_w, FW_T, 0, st>>>(
-            sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, w.act,
-            small ? w.witem : nullptr, w.bin_start, w.wr, w.cw, rl2, self_interaction, jbits, lt,
-            kmax, ld, nbr, count, status);
+      auto launch = [&](auto kern) {
+        kern<<<g_w, FW_T, 0, st>>>(sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, w.act,
+                                   small ? w.witem : nullptr, w.bin_start, w.wr, w.cw, rl2,
+                                   self_interaction, jbits, lt, kmax, ld, nbr, count, status);
+      };
+      if (narrow && dense_bins)
+        launch(k_fill_wave<true, FW_C_DENSE>);
+      else if (narrow)
+        launch(k_fill_wave<true, FW_C_SPARSE>);
+      else if (dense_bins)
+        launch(k_fill_wave<false, FW_C_DENSE>);
       else
-        k_fill_wave<false><<<g_w, FW_T, 0, st>>>(
-            sys_start, n_systems, w.geom, w.grid, w.meta, bin_cap, w.act,
-            small ? w.witem : nullptr, w.bin_start, w.wr, w.cw, rl2, self_interaction, jbits, lt,
-            kmax, ld, nbr, count, status);
+        launch(k_fill_wave<false, FW_C_SPARSE>);
       TSX_CHECK_LAUNCH();
       return TSX_OK;
     }
