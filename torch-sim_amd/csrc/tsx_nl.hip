@@ -641,8 +641,11 @@ __global__ __launch_bounds__(256) void k_scatter(
 // order, wrapped coordinates with the packed wrap shift, the partial list
 // word, and the bin-relative frame (anchor, rel) when asked for.  Returns the
 // atom's largest wrap magnitude.
+// rloc: the atom's position record (x, y, z, w) already in registers, or null
+// to read it from pos
 __device__ __forceinline__ int emit_atom(int i, int dst, int s, int lb, const WrapOut& o,
                                          const double* __restrict__ pos, int ps,
+                                         const double* rloc,
                                          const int32_t* __restrict__ sys_start, const double* g,
                                          const int32_t* gr, int jbits, int sbits, int stencil,
                                          int32_t* __restrict__ order, double* __restrict__ wr,
@@ -659,7 +662,7 @@ __device__ __forceinline__ int emit_atom(int i, int dst, int s, int lb, const Wr
   cw[dst] = (uint32_t)(i - sys_start[s]) |
             ((stencil ? (uint32_t)bz : pack_fields(-o.w[0], -o.w[1], -o.w[2], sbits)) << jbits);
   if (anchor || rel) {
-    const double* r = pos + (int64_t)ps * i;
+    const double* r = rloc ? rloc : pos + (int64_t)ps * i;
     double A[3];
     if (g[22] == 0.0) {  // no lattice: anchor at the system's first atom
       const double* r0 = pos + (int64_t)ps * sys_start[s];
@@ -726,7 +729,7 @@ __global__ __launch_bounds__(256) void k_sort_atom(
       dst = lo + rank;
     }
     const WrapOut o = wrap_atom(pos + (int64_t)ps * i, g, gr[7]);
-    wm = max(wm, emit_atom(i, dst, s, b - gr[6], o, pos, ps, sys_start, g, gr, jbits, sbits,
+    wm = max(wm, emit_atom(i, dst, s, b - gr[6], o, pos, ps, nullptr, sys_start, g, gr, jbits, sbits,
                            stencil, order, wr, cw, anchor, rel));
     bin_count[b] = 0;
   }
@@ -853,6 +856,21 @@ __global__ __launch_bounds__(SB_THREADS) void k_build_small(
   const int32_t* gr = grid + (size_t)GRID * s;
   const double* g = geom + (size_t)GEOM * s;
   const int base = sys_start[s], n_s = sys_start[s + 1] - base;
+  // This thread's atoms (a = threadIdx.x + u SB_THREADS, at most two): their
+  // position records are loaded once, here, so that this round trip overlaps
+  // the work-item prefix's, and the rank pass below reuses them from registers.
+  static_assert(SB_MAXN <= 2 * SB_THREADS, "at most two atoms per thread");
+  double rr[2][4];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int a = threadIdx.x + u * SB_THREADS;
+    for (int k = 0; k < 4; ++k) rr[u][k] = 0.0;
+    if (a < n_s && n_s <= SB_MAXN) {  // an empty system reads nothing
+      const double* r = pos + (int64_t)ps * (base + a);
+      for (int k = 0; k < 3; ++k) rr[u][k] = r[k];
+      if (ps == 4) rr[u][3] = r[3];
+    }
+  }
   if (witem) {
     // this system's work items of the per-bin fill, written before any
     // early exit so that [0, act[n_sys]) has no holes: its active-bin prefix
@@ -886,20 +904,23 @@ __global__ __launch_bounds__(SB_THREADS) void k_build_small(
     for (int k = threadIdx.x; k <= nb; k += SB_THREADS) lcnt[k] = 0;
   if (threadIdx.x == 0) s_wm = 0;
   __syncthreads();
-  for (int a = threadIdx.x; a < n_s; a += SB_THREADS) {
+  WrapOut ow[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int a = threadIdx.x + u * SB_THREADS;
+    if (a >= n_s) break;
     const int i = base + a;
-    const double* r = pos + (int64_t)ps * i;
-    const WrapOut o = wrap_atom(r, g, gr[7]);
-    const int b = bin_of(o, gr);
+    ow[u] = wrap_atom(rr[u], g, gr[7]);
+    const int b = bin_of(ow[u], gr);
     abin[a] = b - off;
     aslot[a] = atomicAdd(&cnt[b - off], 1);
     atom_bin[i] = b;
     if (xref) {
       double4 v;
-      v.x = r[0];
-      v.y = r[1];
-      v.z = r[2];
-      v.w = ps == 4 ? r[3] : 0.0;
+      v.x = rr[u][0];
+      v.y = rr[u][1];
+      v.z = rr[u][2];
+      v.w = rr[u][3];
       reinterpret_cast<double4*>(xref)[i] = v;
     }
   }
@@ -912,15 +933,17 @@ __global__ __launch_bounds__(SB_THREADS) void k_build_small(
   for (int a = threadIdx.x; a < n_s; a += SB_THREADS) uns[cnt[abin[a]] + aslot[a]] = a;
   __syncthreads();
   int wm = 0;
-  for (int a = threadIdx.x; a < n_s; a += SB_THREADS) {
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int a = threadIdx.x + u * SB_THREADS;
+    if (a >= n_s) break;
     const int lb = abin[a];
     const int lo = cnt[lb], hi = lb + 1 < nb ? cnt[lb + 1] : n_s;
     int rank = 0;
     for (int k = lo; k < hi; ++k) rank += uns[k] < a;
     const int i = base + a;
-    const WrapOut o = wrap_atom(pos + (int64_t)ps * i, g, gr[7]);
-    wm = max(wm, emit_atom(i, base + lo + rank, s, lb, o, pos, ps, sys_start, g, gr, jbits, sbits,
-                           stencil, order, wr, cw, anchor, rel));
+    wm = max(wm, emit_atom(i, base + lo + rank, s, lb, ow[u], pos, ps, rr[u], sys_start, g, gr,
+                           jbits, sbits, stencil, order, wr, cw, anchor, rel));
   }
   for (int o2 = 32; o2 > 0; o2 >>= 1) wm = max(wm, __shfl_xor(wm, o2, 64));
   if ((threadIdx.x & 63) == 0) atomicMax(&s_wm, wm);
