@@ -124,7 +124,7 @@ struct FusedArgs {
 struct ExactCtx {
   const double4* pos_s;  // this system's packed positions
   const double4* anc_s;  // this system's anchors
-  double xi, yi, zi;     // r_i - A_i
+  int il;                // this row's atom within the system (r_i - A_i when needed)
   const double* u;       // ustep row of the system
   double rc2;
 };
@@ -230,7 +230,7 @@ __device__ __forceinline__ float4 buf_rel(__amdgpu_buffer_rsrc_t r, uint32_t vof
 // Byte offsets are 32-bit: the caller guarantees 4 * (rows * TPA) and
 // 4 * 7 * ld * TPA below 2^32 and 16 * N below 2^32.
 template <int KIND, int TPA, bool VIRIAL, bool MULTI, bool ORTHO, bool NTW>
-__device__ __forceinline__ RowAcc accumulate_rel(const PairP* sp, int ns, int ti,
+__device__ __forceinline__ RowAcc accumulate_rel(const PairP* sp, const PairP& p0, int ns, int ti,
                                                  const float* __restrict__ rel_all,
                                                  uint32_t rel_base, float xri, float yri,
                                                  float zri, const float* uf,
@@ -242,7 +242,6 @@ __device__ __forceinline__ RowAcc accumulate_rel(const PairP* sp, int ns, int ti
   for (int c = 0; c < 6; ++c) a.w[c] = 0.0;
   const int sbits = (32 - jbits) / 3;
   const uint32_t jmask = (1u << jbits) - 1u;
-  const PairP p0 = sp[MULTI ? 0 : ti * ns];
   const int64_t stride = ld * TPA;
   const int iters = cnt > lane ? (cnt - lane + TPA - 1) / TPA : 0;
   const __amdgpu_buffer_rsrc_t rrel = buf_rsrc(rel_all);
@@ -287,6 +286,9 @@ __device__ __forceinline__ RowAcc accumulate_rel(const PairP* sp, int ns, int ti
       add_pair<VIRIAL>(a, in ? e : 0.f, in ? fs : 0.f, dx, dy, dz);
     }
     while (amb) {  // rare: the FP64 displacement decides
+      const double4 xi4 = ex.pos_s[ex.il];
+      const double4 ai4 = ex.anc_s[ex.il];
+      const double xi = xi4.x - ai4.x, yi = xi4.y - ai4.y, zi = xi4.z - ai4.z;
       const int u = __builtin_ctz(amb);
       amb &= amb - 1u;
       uint32_t wu = w[0];
@@ -298,9 +300,9 @@ __device__ __forceinline__ RowAcc accumulate_rel(const PairP* sp, int ns, int ti
       const double iz = (double)nl_shift(wu, jbits, sbits, 2);
       const double4 pj = ex.pos_s[jl];
       const double4 aj = ex.anc_s[jl];
-      const double gx = ((pj.x - aj.x) - ex.xi) + fma(iz, ex.u[6], fma(iy, ex.u[3], ix * ex.u[0]));
-      const double gy = ((pj.y - aj.y) - ex.yi) + fma(iz, ex.u[7], fma(iy, ex.u[4], ix * ex.u[1]));
-      const double gz = ((pj.z - aj.z) - ex.zi) + fma(iz, ex.u[8], fma(iy, ex.u[5], ix * ex.u[2]));
+      const double gx = ((pj.x - aj.x) - xi) + fma(iz, ex.u[6], fma(iy, ex.u[3], ix * ex.u[0]));
+      const double gy = ((pj.y - aj.y) - yi) + fma(iz, ex.u[7], fma(iy, ex.u[4], ix * ex.u[1]));
+      const double gz = ((pj.z - aj.z) - zi) + fma(iz, ex.u[8], fma(iy, ex.u[5], ix * ex.u[2]));
       const double d2e = gx * gx + gy * gy + gz * gz;
       if (!(d2e < ex.rc2)) continue;
       float e, fs;
@@ -325,37 +327,45 @@ __device__ __forceinline__ RowAcc accumulate_rel(const PairP* sp, int ns, int ti
   return a;
 }
 
-// LJ and Morse without the virial: four waves per SIMD.  Their unconstrained
-// allocation lands at 130-132 VGPRs (three waves), and the gather-bound walk
-// gains more from the fourth wave's loads in flight than it loses to the
-// tighter budget (they fit it without spills).  The virial and soft-sphere
-// variants keep the default budget: capped, they would spill.
+// LJ and Morse without the virial: four waves per SIMD (107-111 VGPRs with
+// one species pair, whose constants stay in registers).  The gather-bound
+// walk gains from the fourth wave's loads in flight; a fifth (96 VGPRs, a
+// few spills outside the walk) loses more to the scheduler's tighter
+// budget than it gains.  The virial and soft-sphere variants keep the
+// default budget: capped, they would spill.
 template <int KIND, int TPA, bool LANGEVIN, bool VIRIAL, bool MULTI, bool ORTHO, bool NTW = false>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(VIRIAL || KIND == TSX_SOFT_SPHERE ? 1 : 4)))
 void k_md_fused(FusedArgs A) {
-  __shared__ PairP sp[MAX_SPECIES_MD * MAX_SPECIES_MD];
-  for (int k = threadIdx.x; k < A.ns * A.ns; k += blockDim.x)
-    sp[k] = make_pair_params(KIND, A.params[3 * k], A.params[3 * k + 1], A.params[3 * k + 2]);
-  __syncthreads();
+  // Prologue chain before the first neighbour gather: row -> (atom, system)
+  // -> (system start, frame, own relative position).  Rows of a system cover
+  // its own index range [sys_start[s], sys_start[s + 1]), so atom_sys[t] is
+  // the row's system without waiting for order[t]; rel_in[i] holds
+  // (float)(pos_in[i] - anchor[i]) exactly (written by the build or the
+  // previous step from the same FP64 difference), so the FP64 records are
+  // read only by the tail and the rare FP64 cutoff path.
   const int64_t gid = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   const int64_t t = gid / TPA;
   const int lane = (int)(gid % TPA);
   const bool valid = t < A.n;
   const int64_t tt = valid ? t : 0;
   const int i = A.order[tt];
-  const int s = A.atom_sys[i];
+  const int s = A.atom_sys[tt];
   const int base = A.sys_start[s];
-  int ti;
-  ExactCtx ex;
-  {
-    const double4 x0 = reinterpret_cast<const double4*>(A.pos_in)[i];
-    const double4 a0 = reinterpret_cast<const double4*>(A.anchor)[i];
-    ti = MULTI ? (int)__double_as_longlong(x0.w) : 0;
-    ex.xi = x0.x - a0.x;
-    ex.yi = x0.y - a0.y;
-    ex.zi = x0.z - a0.z;
+  const float4 ri = reinterpret_cast<const float4*>(A.rel_in)[i];
+  const int ti = MULTI ? __float_as_int(ri.w) : 0;
+  // one species pair: its constants in registers; several: an LDS table
+  __shared__ PairP sp[MAX_SPECIES_MD * MAX_SPECIES_MD];
+  PairP pp{};
+  if (MULTI) {
+    for (int k = threadIdx.x; k < A.ns * A.ns; k += blockDim.x)
+      sp[k] = make_pair_params(KIND, A.params[3 * k], A.params[3 * k + 1], A.params[3 * k + 2]);
+    __syncthreads();
+  } else {
+    pp = make_pair_params(KIND, A.params[0], A.params[1], A.params[2]);
   }
+  ExactCtx ex;
+  ex.il = i - base;
   // the system's bin steps and the FP32 rounding margin around rc: every
   // term of the FP32 displacement is bounded by M = rc + 2 skin + 3 sum|u_k|
   // and the displacement carries fewer than 32 roundings of 2^-24 M
@@ -373,8 +383,8 @@ void k_md_fused(FusedArgs A) {
   ex.rc2 = A.rc * A.rc;
   const int cnt = valid ? A.count[tt] : 0;
   const RowAcc a = accumulate_rel<KIND, TPA, VIRIAL, MULTI, ORTHO, NTW>(
-      sp, A.ns, ti, A.rel_in, (uint32_t)base, (float)ex.xi, (float)ex.yi, (float)ex.zi, uf,
-      A.nbr, tt, cnt, lane, A.jbits, A.ld, lo2, hi2, ex);
+      sp, pp, A.ns, ti, A.rel_in, (uint32_t)base, ri.x, ri.y, ri.z, uf, A.nbr, tt, cnt, lane,
+      A.jbits, A.ld, lo2, hi2, ex);
   // Philox normals of (seed, step, atom) for the O step: with two or more
   // lanes per atom, lanes 0 and 1 each draw one of normal3's two blocks (the
   // same operations on each block, so the same values), in one instruction
