@@ -1544,7 +1544,11 @@ void k_fill_wave(
   __shared__ uint32_t s_sw[FW_W][FW_IMG];
   __shared__ uint8_t s_cimg[FW_W][FW_CH];
   if (meta[0] > bin_cap) return;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  // the wave's index, wave-uniform by construction: stated so that the work
+  // items, bin records and centre atoms derived from it load through the
+  // scalar cache into SGPRs
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const WordCtx wc = word_ctx(jbits);
   const int64_t stride = ld << lt;
   const int tmask = (1 << lt) - 1;
