@@ -1528,8 +1528,12 @@ constexpr double FW_DENSE_OCC = 3.0;  // mean atoms per bin from which NC is 6
 // NARROW: row slots addressed by a 24-bit multiply from a per-centre row
 // pointer (ld tpa below 2^24 and every slot index of a row below 2^31);
 // otherwise by 64-bit arithmetic
+// NC = 4 (sparse bins, one large box) at seven waves per SIMD (72 VGPRs, the
+// LDS limit); NC = 6 keeps the compiler's 74 (six waves), where the seventh
+// wave cost the dense Cu batch 4 %.
 template <bool NARROW, int NC>
 __global__ __launch_bounds__(FW_T)
+__attribute__((amdgpu_waves_per_eu(NC == FW_C_SPARSE ? 7 : 1)))
 void k_fill_wave(
     const int32_t* __restrict__ sys_start, int n_sys, const double* __restrict__ geom,
     const int32_t* __restrict__ grid, const int32_t* __restrict__ meta, int bin_cap,
