@@ -132,17 +132,17 @@ const char* tsx_last_error(void);
  * gated rebuilds); TSX_FILL_AUTO picks BIN for bins_per_rlist >= 2,
  * kmax >= 64 and more than one system, and WAVE for stencil words with
  * bins_per_rlist >= 2 and kmax >= 64 (one large system included).  WAVE
- * tests up to 6 centre atoms per pass over the candidates when n_atoms is at
- * least 3 bin_cap (dense bins), else 4; TSX_FILL_WAVE4 / TSX_FILL_WAVE6 fix
- * that width.
+ * tests up to 8 centre atoms per pass over the candidates when n_atoms is at
+ * least 3 bin_cap (dense bins), else 4; TSX_FILL_WAVE_SPARSE /
+ * TSX_FILL_WAVE_DENSE fix that width.
  * All write every row in the same order (stencil image, then atom index).
  */
 #define TSX_FILL_AUTO 0
 #define TSX_FILL_BIN 1
 #define TSX_FILL_ATOM 2
 #define TSX_FILL_WAVE 3
-#define TSX_FILL_WAVE4 4
-#define TSX_FILL_WAVE6 5
+#define TSX_FILL_WAVE_SPARSE 4
+#define TSX_FILL_WAVE_DENSE 5
 
 /* ``word_mode``: TSX_WORDS_SHIFT writes the image shift S' in the three word
  * fields (the layout above, read by every consumer of the reference triple);

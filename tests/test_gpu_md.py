@@ -264,7 +264,7 @@ def test_engine_batch_equals_single_bitwise_with_rebuilds(dev):
 
 def test_engine_fill_kernels_give_the_same_trajectory(dev):
     """Hot Cu (2000 K) with a small skin rebuilds many systems' lists; the
-    per-bin, per-atom and one-wave-per-bin fills (4 or 6 centres per pass)
+    per-bin, per-atom and one-wave-per-bin fills (4 or 8 centres per pass)
     write the same rows, so the trajectories and energies are bitwise the
     same, and so are consecutive runs against one run of their total length
     (the engine leaves the last step open)."""

@@ -158,10 +158,10 @@ def test_fill_kernels_write_identical_rows(dev, q):
 @pytest.mark.parametrize("q", [1, 2, 3])
 def test_wave_fill_writes_the_same_stencil_rows(dev, q):
     """Stencil-word lists (the MD engine's): the one-wave-per-bin fill (mode
-    3, and modes 4 / 5 with 4 / 6 centre atoms per pass) writes every row
+    3, and modes 4 / 5 with 4 / 8 centre atoms per pass) writes every row
     entry for entry as the per-bin and per-atom fills do, on dense Cu
-    systems (bins of up to 10 atoms: one and two passes of 6), small cells
-    with many periodic images and triclinic soft-sphere cells."""
+    systems (bins of up to 10 atoms: one and two passes of either width),
+    small cells with many periodic images and triclinic soft-sphere cells."""
     st = tsa.workloads.cu_batch(6, device="cpu", seed=8)
     small = tsa.workloads.cu_batch(3, device="cpu", reps=[[2, 2, 2], [1, 1, 1], [1, 2, 3]], seed=2)
     tri = tsa.workloads.soft_sphere_batch(4, n_min=40, n_max=120, device="cpu", seed=3)

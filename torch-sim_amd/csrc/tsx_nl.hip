@@ -1517,19 +1517,19 @@ constexpr int FW_W = FW_T / 64;
 constexpr int FW_IMG = 128;   // stencil images per batch (two per lane)
 constexpr int FW_CH = 1024;   // candidates per image-table chunk
 // Centre atoms per pass over the candidates (NC): a bin with more atoms walks
-// its stencil's candidates again.  4 suits sparse bins; 6 suits dense ones,
+// its stencil's candidates again.  4 suits sparse bins; 8 suits dense ones,
 // where fcc atoms on bin faces and corners make a third of the bins hold 5
-// or more (one unit cell per bin: 4 atoms on average, up to 10), at 123
-// instead of 117 VGPRs (still four waves per SIMD).
+// or more (one unit cell per bin: 4 atoms on average, up to 10): 1.003
+// passes per bin against 1.36 with 4 (and 1.05 with 6).
 constexpr int FW_C_SPARSE = 4;
-constexpr int FW_C_DENSE = 6;
-constexpr double FW_DENSE_OCC = 3.0;  // mean atoms per bin from which NC is 6
+constexpr int FW_C_DENSE = 8;
+constexpr double FW_DENSE_OCC = 3.0;  // mean atoms per bin from which NC is 8
 
 // NARROW: row slots addressed by a 24-bit multiply from a per-centre row
 // pointer (ld tpa below 2^24 and every slot index of a row below 2^31);
 // otherwise by 64-bit arithmetic
 // NC = 4 (sparse bins, one large box) at seven waves per SIMD (72 VGPRs, the
-// LDS limit); NC = 6 keeps the compiler's 74 (six waves), where the seventh
+// LDS limit); NC = 8 keeps the compiler's 76 (six waves), where a seventh
 // wave cost the dense Cu batch 4 %.
 template <bool NARROW, int NC>
 __global__ __launch_bounds__(FW_T)
@@ -2019,7 +2019,7 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
   TSX_REQUIRE(r_list > 0.0, "r_list must be positive");
   TSX_REQUIRE(bins_per_rlist >= 1 && bins_per_rlist <= 4, "bins_per_rlist in [1,4]");
   TSX_REQUIRE(bin_cap >= 1, "bin_cap must be >= 1");
-  TSX_REQUIRE(fill_mode >= TSX_FILL_AUTO && fill_mode <= TSX_FILL_WAVE6, "unknown fill_mode");
+  TSX_REQUIRE(fill_mode >= TSX_FILL_AUTO && fill_mode <= TSX_FILL_WAVE_DENSE, "unknown fill_mode");
   TSX_REQUIRE(word_mode == TSX_WORDS_SHIFT || word_mode == TSX_WORDS_STENCIL,
               "unknown word_mode");
   TSX_REQUIRE(word_mode == TSX_WORDS_SHIFT || (!half && jbits <= 22),
@@ -2118,8 +2118,8 @@ int tsx_nl_build(const double* pos, int32_t pos_stride, const double* cell_col,
     if (per_wave) {
       // centres per pass from the mean bin occupancy (bin_cap is the caller's
       // bin count plus a margin) unless the mode names it
-      const bool dense_bins = fill_mode == TSX_FILL_WAVE6 ||
-                              (fill_mode != TSX_FILL_WAVE4 &&
+      const bool dense_bins = fill_mode == TSX_FILL_WAVE_DENSE ||
+                              (fill_mode != TSX_FILL_WAVE_SPARSE &&
                                (double)n_atoms >= FW_DENSE_OCC * (double)bin_cap);
       const int g_res_w = dense_bins ? fill_wave_resident_blocks<FW_C_DENSE>()
                                      : fill_wave_resident_blocks<FW_C_SPARSE>();

@@ -189,7 +189,7 @@ class NeighborList:
         one_block_build: bool = True,
     ):
         self.r_list = float(r_list)
-        self.fill_mode = int(fill_mode)  # TSX_FILL_AUTO / _BIN / _ATOM / _WAVE / _WAVE4 / _WAVE6
+        self.fill_mode = int(fill_mode)  # TSX_FILL_AUTO / _BIN / _ATOM / _WAVE / _WAVE_SPARSE / _WAVE_DENSE
         self.word_mode = int(word_mode)  # TSX_WORDS_SHIFT / _STENCIL
         # batches of systems of <= 2048 atoms: bin / scan / sort by one
         # workgroup per system (False: the grid-wide kernels)
