@@ -1529,8 +1529,9 @@ constexpr double FW_DENSE_OCC = 3.0;  // mean atoms per bin from which NC is 8
 // pointer (ld tpa below 2^24 and every slot index of a row below 2^31);
 // otherwise by 64-bit arithmetic
 // NC = 4 (sparse bins, one large box) at seven waves per SIMD (72 VGPRs, the
-// LDS limit); NC = 8 keeps the compiler's 76 (six waves), where a seventh
-// wave cost the dense Cu batch 4 %.
+// LDS limit); NC = 8 keeps the compiler's allocation (86 VGPRs with two
+// candidates per trip: five waves), where forcing six or seven waves cost
+// the dense Cu batch 2-4 %.
 template <bool NARROW, int NC>
 __global__ __launch_bounds__(FW_T)
 __attribute__((amdgpu_waves_per_eu(NC == FW_C_SPARSE ? 7 : 1)))
@@ -1706,35 +1707,79 @@ void k_fill_wave(
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
           __builtin_amdgcn_wave_barrier();
           __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-          for (int cb = 0; cb < nk; cb += 64) {  // uniform trips over the wave
-            const Cand cd = cand_load(cb + lane, nk, k0);
-            const bool valid = cb + lane < nk;
-            const double sx = s_ics[w][cd.q][0], sy = s_ics[w][cd.q][1], sz = s_ics[w][cd.q][2];
-            const uint32_t swq = s_sw[w][cd.q];
-            const int jl = (int)(cd.cwj & wc.jmask);
-            const uint32_t word = (uint32_t)jl | swq;
+          if (NC == FW_C_DENSE) {
+            // uniform trips over the wave, two candidates per lane: both
+            // records in flight together, tested in candidate order (86
+            // VGPRs, five waves per SIMD: +0.8 % on the dense Cu batch)
+            for (int cb = 0; cb < nk; cb += 128) {
+              Cand cds[2];
+              cds[0] = cand_load(cb + lane, nk, k0);
+              cds[1] = cand_load(cb + 64 + lane, nk, k0);
 #pragma unroll
-            for (int a = 0; a < NC; ++a) {
-              if (a >= na) break;  // uniform over the wave
-              const double d2 = dist2_ref(cx[a], cy[a], cz[a], cd.wj.x, cd.wj.y, cd.wj.z, sx, sy, sz);
-              const bool match = valid && d2 < rl2 && (self_int || !(swq == 0 && jl == il[a]));
-              const uint64_t bal = __ballot(match);
-              if (match) {
-                const int pos = cnt[a] + (int)__builtin_amdgcn_mbcnt_hi(
-                                             (uint32_t)(bal >> 32),
-                                             __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                if (pos < kmax) {
-                  if (NARROW) {
-                    uint32_t* rowp = nbr + ((int64_t)(a0 + a) << lt);  // wave-uniform
-                    rowp[__umul24((uint32_t)(pos >> lt), (uint32_t)stride) +
-                         (uint32_t)(pos & tmask)] = word;
-                  } else {
-                    nbr[((int64_t)(a0 + a) << lt) + (int64_t)(pos >> lt) * stride +
-                        (pos & tmask)] = word;
+              for (int h = 0; h < 2; ++h) {
+                if (h == 1 && cb + 64 >= nk) break;  // uniform over the wave
+                const Cand& cd = cds[h];
+                const bool valid = cb + 64 * h + lane < nk;
+                const double sx = s_ics[w][cd.q][0], sy = s_ics[w][cd.q][1], sz = s_ics[w][cd.q][2];
+                const uint32_t swq = s_sw[w][cd.q];
+                const int jl = (int)(cd.cwj & wc.jmask);
+                const uint32_t word = (uint32_t)jl | swq;
+#pragma unroll
+                for (int a = 0; a < NC; ++a) {
+                  if (a >= na) break;  // uniform over the wave
+                  const double d2 = dist2_ref(cx[a], cy[a], cz[a], cd.wj.x, cd.wj.y, cd.wj.z, sx, sy, sz);
+                  const bool match = valid && d2 < rl2 && (self_int || !(swq == 0 && jl == il[a]));
+                  const uint64_t bal = __ballot(match);
+                  if (match) {
+                    const int pos = cnt[a] + (int)__builtin_amdgcn_mbcnt_hi(
+                                                 (uint32_t)(bal >> 32),
+                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                    if (pos < kmax) {
+                      if (NARROW) {
+                        uint32_t* rowp = nbr + ((int64_t)(a0 + a) << lt);  // wave-uniform
+                        rowp[__umul24((uint32_t)(pos >> lt), (uint32_t)stride) +
+                             (uint32_t)(pos & tmask)] = word;
+                      } else {
+                        nbr[((int64_t)(a0 + a) << lt) + (int64_t)(pos >> lt) * stride +
+                            (pos & tmask)] = word;
+                      }
+                    }
                   }
+                  cnt[a] += __popcll(bal);
                 }
               }
-              cnt[a] += __popcll(bal);
+            }
+          } else {
+            for (int cb = 0; cb < nk; cb += 64) {  // uniform trips over the wave
+              const Cand cd = cand_load(cb + lane, nk, k0);
+              const bool valid = cb + lane < nk;
+              const double sx = s_ics[w][cd.q][0], sy = s_ics[w][cd.q][1], sz = s_ics[w][cd.q][2];
+              const uint32_t swq = s_sw[w][cd.q];
+              const int jl = (int)(cd.cwj & wc.jmask);
+              const uint32_t word = (uint32_t)jl | swq;
+#pragma unroll
+              for (int a = 0; a < NC; ++a) {
+                if (a >= na) break;  // uniform over the wave
+                const double d2 = dist2_ref(cx[a], cy[a], cz[a], cd.wj.x, cd.wj.y, cd.wj.z, sx, sy, sz);
+                const bool match = valid && d2 < rl2 && (self_int || !(swq == 0 && jl == il[a]));
+                const uint64_t bal = __ballot(match);
+                if (match) {
+                  const int pos = cnt[a] + (int)__builtin_amdgcn_mbcnt_hi(
+                                               (uint32_t)(bal >> 32),
+                                               __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                  if (pos < kmax) {
+                    if (NARROW) {
+                      uint32_t* rowp = nbr + ((int64_t)(a0 + a) << lt);  // wave-uniform
+                      rowp[__umul24((uint32_t)(pos >> lt), (uint32_t)stride) +
+                           (uint32_t)(pos & tmask)] = word;
+                    } else {
+                      nbr[((int64_t)(a0 + a) << lt) + (int64_t)(pos >> lt) * stride +
+                          (pos & tmask)] = word;
+                    }
+                  }
+                }
+                cnt[a] += __popcll(bal);
+              }
             }
           }
           // every lane's reads of this chunk's table are done before it is rewritten
