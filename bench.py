@@ -65,17 +65,14 @@ OVERHEAD_BYTES = 84.0
 # (bench.py --calibrate-copy 256: FETCH_SIZE x 2.000, WRITE_SIZE x 1.000).
 # Keyed by (workload, skin, tpa); "source" names the summary with the commands.
 MEASURED_TRAFFIC = {
-    ("cu_morse_nvt", 0.48, 2): {"fetch_kib": 109179.5, "fetch_factor": 2.000,
-                                "write_kib": 42419.1, "write_factor": 1.000,
+    ("cu_morse_nvt", 0.48, 2): {"fetch_kib": 102975.7, "fetch_factor": 2.000,
+                                "write_kib": 42704.6, "write_factor": 1.000,
                                 "source": "profiles/k_md_fused__cu_morse_nvt_512sys.md"},
-    ("cu_morse_nvt", 0.6, 2): {"fetch_kib": 115619.0, "fetch_factor": 2.000,
-                               "write_kib": 42414.5, "write_factor": 1.000,
+    ("cu_morse_nvt", 0.6, 2): {"fetch_kib": 109373.6, "fetch_factor": 2.000,
+                               "write_kib": 42558.4, "write_factor": 1.000,
                                "source": "profiles/k_md_fused__cu_morse_nvt_512sys.md"},
-    ("cu_morse_nvt", 0.66, 2): {"fetch_kib": 119659.0, "fetch_factor": 2.000,
-                                "write_kib": 42546.3, "write_factor": 1.000,
-                                "source": "profiles/k_md_fused__cu_morse_nvt_512sys.md"},
-    ("lj1m_nve", 0.3, 4): {"fetch_kib": 326511.6, "fetch_factor": 2.000,
-                           "write_kib": 126562.4, "write_factor": 1.000,
+    ("lj1m_nve", 0.3, 4): {"fetch_kib": 319485.7, "fetch_factor": 2.000,
+                           "write_kib": 126905.9, "write_factor": 1.000,
                            "source": "profiles/k_md_fused__lj_fluid_1m_nve.md"},
 }
 
