@@ -706,7 +706,7 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-legs", action="store_true",
                     help="headline workload only (no LJ-1M / FIRE / fit legs)")
-    ap.add_argument("--cpu-steps", type=int, default=2,
+    ap.add_argument("--cpu-steps", type=int, default=5,
                     help="BAOAB steps of the full batch in the CPU baseline")
     ap.add_argument("--fit-md-steps", type=int, default=200)
     ap.add_argument("--fire-queue", type=int, default=4096)
