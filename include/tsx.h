@@ -331,10 +331,14 @@ int tsx_md_baoa(double* x, double* p, const double* f, const double* m, const in
  * ``step`` is a device counter giving the Philox step of the next step's
  * noise.  w_atom may be NULL.
  * The list must be a TSX_WORDS_STENCIL list with its frame (anchor, ustep and
- * rel_in = the rel of pos_in): each neighbour costs one 16-byte gather of
- * rel; the FP32 displacement decides the cutoff outside a proven rounding
- * margin and the FP64 displacement (r_j - A_j) - (r_i - A_i) + d.ustep
- * decides inside it.  rel_out receives the rel of pos_out.
+ * rel_in = the rel of pos_in, i.e. (float)(pos_in - anchor) per component,
+ * exactly as tsx_nl_build and the previous fused step write it; the centre
+ * atom's FP32 position is read from it too): each neighbour costs one
+ * 16-byte gather of rel; the FP32 displacement decides the cutoff outside a
+ * proven rounding margin and the FP64 displacement (r_j - A_j) - (r_i - A_i)
+ * + d.ustep decides inside it.  rel_out receives the rel of pos_out.
+ * ld * tpa must stay below 2^27 and n_atoms below 2^28 (32-bit buffer
+ * offsets of the word and record loads), else TSX_ERR_ARG.
  * ``orthorhombic`` nonzero promises that every system's cell is orthorhombic
  * (the off-diagonal entries of cell_col, hence of ustep, are exactly zero):
  * the stencil shift d.ustep then costs one product per component, with the
