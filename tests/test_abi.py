@@ -1,6 +1,7 @@
 """The C-ABI library loads and exports every entry point declared in
 include/tsx.h (no device work)."""
 
+import ctypes
 import os
 import re
 
@@ -91,3 +92,26 @@ def test_binding_types_match_header():
         if got != want:
             bad[name] = (want, got)
     assert not bad, bad
+
+
+def test_fused_step_refuses_lists_past_its_32bit_offsets():
+    """The fused step checks its arguments before any device work: a list
+    whose word offsets (8 row strides of ld * tpa words) or record offsets
+    (16 bytes per atom) pass 2^32 bytes, or an ld below n_atoms, returns
+    TSX_ERR_ARG with a message instead of launching (tsx.h, fused step)."""
+    lib = tsa._lib.lib()
+    buf = [ctypes.create_string_buffer(64) for _ in range(8)]
+    p = [ctypes.addressof(b) for b in buf]
+
+    def call(n_atoms, ld, tpa):
+        # kind Morse, one species, NVE; pointers are never dereferenced
+        return lib.tsx_md_fused_step(
+            1, p[0], 1, p[1], p[2], None, None, p[3], p[3], p[4], p[3], p[3], n_atoms, 11, tpa,
+            ld, 6.0, p[5], p[6], p[7], p[7], None, 0.002, 0, None, None, None, 0, None, 0, 1,
+            0.3, None, p[1], p[2], p[4], p[5], 1, 0, None)
+
+    assert call(10, 2**27, 2) == -1
+    assert "too large" in lib.tsx_last_error().decode()
+    assert call(10, 5, 2) == -1
+    assert "ld must be at least n_atoms" in lib.tsx_last_error().decode()
+    assert call(2**28, 2**28, 1) == -1
