@@ -141,6 +141,40 @@ def test_lj16k_engine_nve_matches_oracle(dev, lj16k):
     torch.testing.assert_close(out.energy.cpu(), e, rtol=1e-6, atol=0)
 
 
+def test_binary_lj_engine_nve_matches_oracle(dev):
+    """The fused step's several-species variant (per-pair constants in an LDS
+    table, each atom's species from the bits of its frame record) on a batch
+    of binary (Kob-Andersen) LJ liquids: 10 NVE steps against the oracle's
+    velocity Verlet with its species-lookup pair function."""
+    st = tsa.workloads.binary_lj_batch(6, n_min=128, n_max=512, rho=0.8, device=dev, seed=5)
+    sig = torch.tensor([[1.0, 0.8], [0.8, 0.88]], dtype=F64)
+    eps = torch.tensor([[1.0, 1.5], [1.5, 0.5]], dtype=F64)
+    model = tsa.MultiLennardJonesModel(torch.tensor([1, 2]), sig, eps, device=dev, cutoff=2.5)
+    st.rng = 6
+    md = tsa.nve_init(state=st, model=model, kT=0.5)
+    steps, dt = 10, 0.002
+    eng = tsa.BatchedMD(model, md, dt=dt, langevin=False, skin=0.3, tpa=2)
+    eng.run(steps)
+    out = eng.state()
+    cpu = st.to(device="cpu")
+    fn = om.MultiPairFn([1, 2], sig, eps, kind="lj")
+
+    def force(x):
+        o = om.pair_forward(x, cpu.cell, cpu.pbc, cpu.atomic_numbers, cpu.system_idx, fn, 2.5,
+                            compute_stress=False, nl=onl.linked_cell_nl)
+        return o["energy"], o["forces"]
+
+    x, p = md.positions.cpu().clone(), md.momenta.cpu().clone()
+    e, f = force(x)
+    for _ in range(steps):
+        x, p, f, e = omd.nve_step(x, p, f, cpu.masses, force, dt)
+    dx = (out.positions.cpu() - x).abs().max().item()
+    dp = (out.momenta.cpu() - p).abs().max().item()
+    _say(f"binary LJ: positions {dx:.3e}, momenta {dp:.3e}")
+    assert dx <= 2e-5 and dp <= 5e-4
+    torch.testing.assert_close(out.energy.cpu(), e, rtol=1e-6, atol=0)
+
+
 # ------------------------------------------- mixed batch above 2048 atoms
 @pytest.fixture(scope="module")
 def mixed(dev):
